@@ -1,0 +1,547 @@
+// ssw_batch.hip -- batched persistent BFS over subsets (gfx950).
+//
+// Replaces the level loop of SlimGPUQueue.Kernel (GPGPU/GPGPU/GPU/SlimGPUQueue.cs:210-407)
+// and follows the search of SlimCPUUntweaked.ComputeMany
+// (GPGPU/GPGPU/CPU/SlimCPUUntweaked.cs:89-216): BFS from Q over subsets, images
+// under a then b, first-discovery wins, stop at the first singleton.
+//
+// Design (one automaton per thread group, many groups per CU):
+//   * a group is 16, 32 or 64 lanes of one wavefront (no s_barrier, wave
+//     ballots only) or a whole 256-thread workgroup (s_barrier + LDS);
+//   * subset images come from nibble-sliced tables in LDS, 16 u64 entries
+//     per 4 states (a in the low word, b in the high word);
+//   * the visited set is a bitmap of 2^n bits, in LDS when n <= lds_n and in
+//     the group's HBM slot otherwise, tested and set with atomicOr (the
+//     winner is the lane that sees the bit clear);
+//   * the frontier is one append-only arena per group (the FIFO of the CPU
+//     search, processed level by level): its first lds_arena_cap entries in
+//     LDS, the rest in the group's HBM slot; newly discovered subsets are
+//     compacted into it with wavefront ballots and popcount prefix sums;
+//   * groups take automata from a device work queue (one atomicAdd per chunk)
+//     until the batch is exhausted, so long-tail automata (Cerny) do not hold
+//     up the rest of the batch;
+//   * the witness (lexicographically smallest shortest reset word) is read
+//     off the retained levels: a backward pass marks the subsets lying on a
+//     shortest path to a singleton, a forward pass takes letter a whenever
+//     its image is such a subset of the next level.
+#include "ssw_internal.h"
+
+namespace ssw {
+
+// Explicit address spaces: LDS (3) and global (1).  With generic pointers the
+// optimizer merges the LDS and HBM branches of the bitmap test-and-set into
+// one flat atomic.
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+typedef __attribute__((address_space(3))) uint64_t lds_u64;
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) u32x4 lds_u4;
+typedef __attribute__((address_space(1))) uint32_t glb_u32;
+typedef __attribute__((address_space(1))) uint8_t glb_u8;
+
+template <class T>
+__device__ __forceinline__ uint32_t or_wg(T *p, uint32_t v) {
+    return __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__device__ __forceinline__ bool is_single(uint32_t x) { return (x & (x - 1u)) == 0u; }
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// G lanes of one wavefront (G = 16, 32, 64).
+template <int G>
+struct WaveGroup {
+    static constexpr int kSize = G;
+    static constexpr int kScratchBytes = 0;
+    uint32_t lane, rank_, gbase;
+    uint64_t gmask, below;
+    __device__ explicit WaveGroup(void *) {
+        lane = __lane_id();
+        rank_ = lane & (G - 1);
+        gbase = lane & ~(uint32_t)(G - 1);
+        gmask = (G == 64) ? ~0ull : (((1ull << (G & 63)) - 1ull) << gbase);
+        below = ((1ull << lane) - 1ull) & gmask;
+    }
+    __device__ uint32_t rank() const { return rank_; }
+    __device__ void sync() const { wave_sync(); }
+    __device__ bool any(bool p) const { return (__ballot(p) & gmask) != 0; }
+    __device__ void exchange(bool single, bool na, bool nb, bool &any_single, uint32_t &off,
+                             uint32_t &total) const {
+        const uint64_t bs = __ballot(single) & gmask;
+        const uint64_t ba = __ballot(na) & gmask;
+        const uint64_t bb = __ballot(nb) & gmask;
+        any_single = bs != 0;
+        off = __popcll(ba & below) + __popcll(bb & below);
+        total = __popcll(ba) + __popcll(bb);
+    }
+    __device__ uint32_t bcast0(uint32_t v) const { return __shfl(v, (int)gbase, 64); }
+};
+
+// A whole workgroup of NT threads (one group per block).
+template <int NT>
+struct BlockGroup {
+    static constexpr int kSize = NT;
+    static constexpr int kWaves = NT / 64;
+    static constexpr int kScratchBytes = 64;
+    uint32_t *scratch;
+    uint32_t tid, lane, wave;
+    __device__ explicit BlockGroup(void *s) : scratch((uint32_t *)s) {
+        tid = threadIdx.x;
+        lane = __lane_id();
+        wave = threadIdx.x >> 6;
+    }
+    __device__ uint32_t rank() const { return tid; }
+    __device__ void sync() const { __syncthreads(); }
+    __device__ bool any(bool p) const {
+        const uint64_t b = __ballot(p);
+        __syncthreads();
+        if (lane == 0) scratch[wave] = b != 0;
+        __syncthreads();
+        bool r = false;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) r |= scratch[w] != 0;
+        return r;
+    }
+    __device__ void exchange(bool single, bool na, bool nb, bool &any_single, uint32_t &off,
+                             uint32_t &total) const {
+        const uint64_t bs = __ballot(single);
+        const uint64_t ba = __ballot(na);
+        const uint64_t bb = __ballot(nb);
+        const uint64_t below = (1ull << lane) - 1ull;
+        __syncthreads();
+        if (lane == 0) {
+            scratch[wave] = __popcll(ba) + __popcll(bb);
+            scratch[kWaves + wave] = bs != 0;
+        }
+        __syncthreads();
+        uint32_t pre = 0, tot = 0;
+        bool anys = false;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+            const uint32_t c = scratch[w];
+            pre += (w < (int)wave) ? c : 0u;
+            tot += c;
+            anys |= scratch[kWaves + w] != 0;
+        }
+        off = pre + __popcll(ba & below) + __popcll(bb & below);
+        total = tot;
+        any_single = anys;
+    }
+    __device__ uint32_t bcast0(uint32_t v) const {
+        __syncthreads();
+        if (tid == 0) scratch[2 * kWaves] = v;
+        __syncthreads();
+        return scratch[2 * kWaves];
+    }
+};
+
+template <class Grp>
+struct GroupSolver {
+    static constexpr uint32_t G = Grp::kSize;
+    const BatchParams &P;
+    Grp g;
+    // LDS
+    lds_u64 *tab;
+    lds_u32 *bm;
+    lds_u32 *arena;
+    // HBM slot
+    glb_u32 *garena;
+    glb_u32 *gbm;
+    glb_u32 *good;
+    glb_u32 *lvl;
+    // automaton being solved (identical in every lane of the group)
+    int64_t ai;
+    int32_t n;
+    uint32_t Q;
+    int32_t nsl;
+    bool lds_bm;
+    uint32_t lo, hi, wr, rd, level;
+    // work queue
+    int64_t it, it_end;
+    bool taken;
+    // progress
+    uint32_t done_local;
+    uint64_t subsets_local;
+
+    __device__ GroupSolver(const BatchParams &p, void *scratch, uint8_t *gsm, uint8_t *slot)
+        : P(p), g(scratch) {
+        tab = (lds_u64 *)gsm;
+        bm = (lds_u32 *)(gsm + kTableBytes);
+        arena = bm + P.lds_bitmap_words;
+        garena = (glb_u32 *)(slot + P.garena_off);
+        gbm = (glb_u32 *)(slot + P.gbitmap_off);
+        good = (glb_u32 *)(slot + P.good_off);
+        lvl = (glb_u32 *)(slot + P.lvl_off);
+        it = 0;
+        it_end = 0;
+        taken = false;
+        done_local = 0;
+        subsets_local = 0;
+    }
+
+    __device__ __forceinline__ uint32_t aget(uint32_t p) const {
+        return p < P.lds_arena_cap ? arena[p] : garena[p - P.lds_arena_cap];
+    }
+    __device__ __forceinline__ void aput(uint32_t p, uint32_t v) const {
+        if (p < P.lds_arena_cap)
+            arena[p] = v;
+        else
+            garena[p - P.lds_arena_cap] = v;
+    }
+    __device__ __forceinline__ uint64_t image(uint32_t s) const {
+        uint64_t r = 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (k < nsl) r |= tab[(k << 4) | ((s >> (4 * k)) & 15u)];
+        return r;
+    }
+    // test-and-set: true when this lane set the bit first
+    __device__ __forceinline__ bool tas(uint32_t x) const {
+        const uint32_t bit = 1u << (x & 31u);
+        uint32_t old;
+        if (lds_bm)
+            old = or_wg(bm + (x >> 5), bit);
+        else
+            old = or_wg(gbm + (x >> 5), bit);
+        return (old & bit) == 0u;
+    }
+
+    __device__ int64_t next_item() {
+        if (P.single_item) {
+            if (taken) return -1;
+            taken = true;
+            return P.first_item;
+        }
+        if (it >= it_end) {
+            uint32_t base = 0;
+            if (g.rank() == 0) base = atomicAdd(P.work_counter, P.chunk);
+            base = g.bcast0(base);
+            it = base;
+            it_end = (int64_t)base + P.chunk;
+            if (it_end > P.count) it_end = P.count;
+            if (it >= P.count) return -1;
+        }
+        return it++;
+    }
+
+    // Loads automaton `a`; false when its transitions are invalid.
+    __device__ bool init(int64_t a) {
+        ai = a;
+        n = P.uniform_n > 0 ? P.uniform_n : (int32_t)P.n_per[ai];
+        const uint32_t r = g.rank();
+        bool bad = n < 1 || n > 30;
+        // the group's HBM bitmap must hold 2^n bits when n > lds_n
+        if (!bad && !lds_ok_for(n) && (uint64_t)(n >= 5 ? (1u << (n - 5)) : 1u) > P.gbitmap_words)
+            bad = true;
+        if (!bad && P.want_witness && (uint64_t)(n >= 5 ? (1u << (n - 5)) : 1u) > P.good_words)
+            bad = true;
+        if (g.any(bad)) {
+            if (r == 0) write_invalid();
+            return false;
+        }
+        const uint8_t *d = P.delta + (P.offset ? P.offset[ai] : (uint64_t)ai * 2u * (uint64_t)n);
+        nsl = (n + 3) >> 2;
+        bool badd = false;
+        for (int q = (int)r; q < 2 * n; q += (int)G) badd |= d[q] >= n;
+        if (g.any(badd)) {
+            if (r == 0) write_invalid();
+            return false;
+        }
+        Q = (1u << n) - 1u;
+        lds_bm = lds_ok_for(n);
+        if (lds_bm) {
+            const uint32_t words = n >= 5 ? (1u << (n - 5)) : 1u;
+            if (words >= 4) {
+                lds_u4 *b4 = (lds_u4 *)bm;
+                for (uint32_t w = r; w < (words >> 2); w += G) b4[w] = u32x4{0u, 0u, 0u, 0u};
+            } else {
+                for (uint32_t w = r; w < words; w += G) bm[w] = 0;
+            }
+        }
+        for (int e = (int)r; e < nsl * 16; e += (int)G) {
+            const int k = e >> 4, v = e & 15;
+            uint64_t acc = 0;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int q = 4 * k + t;
+                if (q < n && ((v >> t) & 1))
+                    acc |= (1ull << d[q]) | (1ull << (32 + d[n + q]));
+            }
+            tab[e] = acc;
+        }
+        g.sync();
+        if (r == 0) {
+            const uint32_t bit = 1u << (Q & 31u);
+            if (lds_bm)
+                or_wg(bm + (Q >> 5), bit);
+            else
+                or_wg(gbm + (Q >> 5), bit);
+            arena[0] = Q;
+            if (P.want_witness) {
+                lvl[0] = 0;
+                lvl[1] = 1;
+            }
+        }
+        g.sync();
+        lo = 0;
+        hi = 1;
+        wr = 1;
+        rd = 0;
+        level = 0;
+        return true;
+    }
+
+    __device__ bool lds_ok_for(int32_t nn) const { return nn <= P.lds_n; }
+
+    __device__ void write_invalid() const {
+        P.length[ai] = -3;
+        if (P.sync) P.sync[ai] = 0;
+        if (P.expanded) P.expanded[ai] = 0;
+    }
+
+    // One level step over up to G frontier entries.
+    __device__ int step() {
+        const uint64_t cap = (uint64_t)P.lds_arena_cap + P.garena_cap;
+        if ((uint64_t)wr + 2u * G > cap) return kOverflow;
+        if (P.want_witness && level + 2 >= P.lvl_cap) return kOverflow;
+        const uint32_t p = rd + g.rank();
+        const bool valid = p < hi;
+        const uint32_t s = valid ? aget(p) : 0u;
+        const uint64_t im = image(s);
+        const uint32_t a = (uint32_t)im, b = (uint32_t)(im >> 32);
+        // a singleton equal to Q (n = 1) is already visited: never a find,
+        // as in SlimCPUUntweaked.cs:149-151
+        const bool sa = valid && is_single(a) && a != Q;
+        const bool sb = valid && is_single(b) && b != Q;
+        const bool na = valid && !sa && tas(a);
+        const bool nb = valid && !sb && tas(b);
+        bool any_single;
+        uint32_t off, total;
+        g.exchange(sa || sb, na, nb, any_single, off, total);
+        if (na) aput(wr + off, a);
+        if (nb) aput(wr + off + (na ? 1u : 0u), b);
+        wr += total;
+        if (any_single) return kFound;
+        rd += G;
+        if (rd >= hi) {
+            g.sync();
+            ++level;
+            lo = hi;
+            hi = wr;
+            rd = lo;
+            if (P.want_witness && g.rank() == 0) lvl[level + 1] = hi;
+            if (lo == hi) return kNoSync;
+        }
+        return kContinue;
+    }
+
+    __device__ __forceinline__ static bool bit_of(const glb_u32 *m, uint32_t x) {
+        return (m[x >> 5] >> (x & 31u)) & 1u;
+    }
+
+    // Lexicographically smallest shortest reset word from the retained levels.
+    __device__ void witness(int32_t L) {
+        glb_u32 *g0 = good;
+        glb_u32 *g1 = good + P.good_words;
+        const uint32_t r = g.rank();
+        g.sync();
+        // backward: mark the subsets of level j with a letter leading into
+        // the marked part of level j+1 (level L: the singletons)
+        for (int32_t j = L - 1; j >= 0; --j) {
+            const uint32_t s = lvl[j], e = lvl[j + 1];
+            glb_u32 *dst = (j & 1) ? g1 : g0;
+            const glb_u32 *src = (j & 1) ? g0 : g1;
+            for (uint32_t base = s; base < e; base += G) {
+                const uint32_t p = base + r;
+                if (p < e) {
+                    const uint32_t x = aget(p);
+                    const uint64_t im = image(x);
+                    const uint32_t a = (uint32_t)im, b = (uint32_t)(im >> 32);
+                    const bool on = (j == L - 1) ? (is_single(a) || is_single(b))
+                                                 : (bit_of(src, a) || bit_of(src, b));
+                    if (on) or_wg(dst + (x >> 5), 1u << (x & 31u));
+                }
+            }
+            g.sync();
+        }
+        // forward: greedy, letter a whenever it stays on a shortest path
+        glb_u8 *out = (glb_u8 *)(P.witness + (uint64_t)ai * (uint64_t)P.witness_stride);
+        uint32_t cur = Q, acc = 0;
+        for (int32_t k = 0; k < L; ++k) {
+            const uint64_t im = image(cur);
+            const uint32_t a = (uint32_t)im, b = (uint32_t)(im >> 32);
+            bool take_a;
+            if (k == L - 1) {
+                take_a = is_single(a);
+            } else {
+                const glb_u32 *gs = ((k + 1) & 1) ? g1 : g0;
+                take_a = false;
+                if (bit_of(gs, a)) {
+                    const uint32_t s = lvl[k + 1], e = lvl[k + 2];
+                    for (uint32_t base = s; base < e && !take_a; base += G) {
+                        const uint32_t p = base + r;
+                        take_a = g.any(p < e && aget(p) == a);
+                    }
+                }
+            }
+            if (!take_a) acc |= 1u << (k & 7);
+            cur = take_a ? a : b;
+            if ((k & 7) == 7 || k == L - 1) {
+                if (r == 0 && (k >> 3) < P.witness_stride) out[k >> 3] = (uint8_t)acc;
+                acc = 0;
+            }
+        }
+        for (int32_t k = ((L + 7) >> 3) + (int32_t)r; k < P.witness_stride; k += (int32_t)G) out[k] = 0;
+        g.sync();
+        // leave both witness bitmaps clean for the next automaton
+        const uint32_t end = lvl[L];
+        for (uint32_t p = r; p < end; p += G) {
+            const uint32_t x = aget(p);
+            g0[x >> 5] = 0;
+            g1[x >> 5] = 0;
+        }
+        g.sync();
+    }
+
+    __device__ void finish(int code) {
+        const uint32_t r = g.rank();
+        if (code == kOverflow) {
+            if (r == 0) {
+                const uint32_t k = atomicAdd(P.deferred_count, 1u);
+                P.deferred_list[k] = (uint32_t)ai;
+            }
+        } else {
+            const bool found = code == kFound;
+            const int32_t L = found ? (int32_t)level + 1 : -1;
+            const int64_t E = hi;
+            if (r == 0) {
+                P.length[ai] = L;
+                if (P.sync) P.sync[ai] = found ? 1 : 0;
+                if (P.expanded) P.expanded[ai] = E;
+                const int32_t bound = P.bound >= 0 ? P.bound : (n - 1) * (n - 1);
+                if (found && L > bound) atomicMin(P.violation, (unsigned long long)ai);
+            }
+            if (found && P.witness) witness(L);
+            ++done_local;
+            subsets_local += (uint64_t)E;
+        }
+        if (!lds_bm) {
+            g.sync();
+            for (uint32_t p = r; p < wr; p += G) gbm[aget(p) >> 5] = 0;
+        }
+        g.sync();
+        if (done_local >= 256) flush();
+    }
+
+    __device__ void flush() {
+        if (g.rank() == 0 && done_local) {
+            atomicAdd(&P.progress[0], (unsigned long long)done_local);
+            atomicAdd(&P.progress[1], (unsigned long long)subsets_local);
+        }
+        done_local = 0;
+        subsets_local = 0;
+    }
+};
+
+template <class Grp>
+__global__ __launch_bounds__(256) void batch_kernel(BatchParams P) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    constexpr uint32_t G = Grp::kSize;
+    const uint32_t groups_per_block = blockDim.x / G;
+    const uint32_t gib = threadIdx.x / G;
+    const uint32_t per_group = kTableBytes + 4u * P.lds_bitmap_words + 4u * P.lds_arena_cap;
+    uint8_t *gsm = smem + (uint64_t)gib * per_group;
+    void *scratch = smem + (uint64_t)groups_per_block * per_group;
+    const uint64_t slot = (uint64_t)blockIdx.x * groups_per_block + gib;
+    GroupSolver<Grp> S(P, scratch, gsm, P.slot_base + slot * P.slot_bytes);
+
+    bool active = true, need_new = true;
+    for (;;) {
+        if (active && need_new) {
+            const int64_t item = S.next_item();
+            if (item < 0) {
+                active = false;
+            } else {
+                const int64_t a = P.order ? (int64_t)P.order[item] : item;
+                need_new = !S.init(a);
+            }
+        }
+        if (G < 64) {
+            if (__ballot(active) == 0) break;
+        } else if (!active) {
+            break;
+        }
+        if (active && !need_new) {
+            const int code = S.step();
+            if (code != kContinue) {
+                S.finish(code);
+                need_new = true;
+            }
+        }
+    }
+    S.flush();
+}
+
+// Clears `words` u32 words (kernel instead of a memset so the call can sit
+// inside a captured graph).
+__global__ void fill_u32_kernel(uint32_t *p, uint64_t words, uint32_t value) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = value;
+}
+
+}  // namespace ssw
+
+// ---- launch wrappers (host side of this translation unit) -------------------
+
+namespace ssw {
+
+template <class Grp>
+static hipError_t launch_batch_t(const BatchParams &P, int grid, int block, size_t lds,
+                                 hipStream_t st) {
+    hipLaunchKernelGGL(batch_kernel<Grp>, dim3(grid), dim3(block), lds, st, P);
+    return hipGetLastError();
+}
+
+template <class Grp>
+static hipError_t occupancy_t(int block, size_t lds, int *blocks) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, batch_kernel<Grp>, block, lds);
+}
+
+hipError_t launch_batch(int group_lanes, const BatchParams &P, int grid, int block, size_t lds,
+                        hipStream_t st) {
+    switch (group_lanes) {
+        case 16: return launch_batch_t<WaveGroup<16>>(P, grid, block, lds, st);
+        case 32: return launch_batch_t<WaveGroup<32>>(P, grid, block, lds, st);
+        case 64: return launch_batch_t<WaveGroup<64>>(P, grid, block, lds, st);
+        case 256: return launch_batch_t<BlockGroup<256>>(P, grid, block, lds, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t batch_occupancy(int group_lanes, int block, size_t lds, int *blocks) {
+    switch (group_lanes) {
+        case 16: return occupancy_t<WaveGroup<16>>(block, lds, blocks);
+        case 32: return occupancy_t<WaveGroup<32>>(block, lds, blocks);
+        case 64: return occupancy_t<WaveGroup<64>>(block, lds, blocks);
+        case 256: return occupancy_t<BlockGroup<256>>(block, lds, blocks);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+size_t batch_scratch_bytes(int group_lanes) {
+    return group_lanes == 256 ? (size_t)BlockGroup<256>::kScratchBytes : 0;
+}
+
+hipError_t fill_u32(uint32_t *p, uint64_t words, uint32_t value, hipStream_t st) {
+    if (words == 0) return hipSuccess;
+    uint64_t blocks = (words + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3((unsigned)blocks), dim3(256), 0, st, p, words, value);
+    return hipGetLastError();
+}
+
+}  // namespace ssw

@@ -1,0 +1,141 @@
+// ssw_host.h -- host-side state of a solver context (not part of the C ABI).
+#pragma once
+
+#include "ssw.h"
+#include "ssw_internal.h"
+
+#include <atomic>
+#include <cstdarg>
+#include <map>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <tuple>
+#include <vector>
+
+namespace ssw {
+
+constexpr int kMaxLdsN = 16;  // largest n whose visited bitmap (8 KiB) lives in LDS
+
+enum SolveMode { kModeBatch = 0, kModePlain = 1, kModeLarge = 2 };
+
+// layout of the small counters buffer
+constexpr size_t kCtrWork = 0;         // u32 work-queue head
+constexpr size_t kCtrDeferred = 4;     // u32 deferred count
+constexpr size_t kCtrViolation = 8;    // u64 smallest violating index
+constexpr size_t kCtrProgress = 16;    // u64[2] automata done, subsets expanded
+constexpr size_t kCtrLarge = 64;       // device-wide engine scratch counters
+constexpr size_t kCtrBytes = 256;
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+// A batch whose every pointer is a device pointer.
+struct DeviceBatch {
+    int64_t count = 0;
+    int32_t uniform_n = 0;
+    int32_t bound = -1;
+    const uint8_t *n_per = nullptr;
+    const uint64_t *offset = nullptr;
+    const uint8_t *delta = nullptr;
+    int32_t *length = nullptr;
+    uint8_t *sync = nullptr;
+    int64_t *expanded = nullptr;
+    uint8_t *witness = nullptr;
+    int32_t witness_stride = 0;
+};
+
+struct ClassSpec {
+    int nmax = 0;
+    bool lds = true;
+    int64_t items = 0;
+    const uint32_t *order = nullptr;
+    const uint8_t *host_n = nullptr;  // plain path: per-automaton n on the host
+    int plain_n = 0;
+};
+
+struct SlotKey {
+    int G, block, lds_n, nmax, witness, arena_lds, plain;
+    bool operator<(const SlotKey &o) const {
+        return std::tie(G, block, lds_n, nmax, witness, arena_lds, plain) <
+               std::tie(o.G, o.block, o.lds_n, o.nmax, o.witness, o.arena_lds, o.plain);
+    }
+};
+
+}  // namespace ssw
+
+struct ssw_context;
+
+namespace ssw {
+
+// Device-wide level loop for large instances (ssw_large.cpp).
+struct LargeEngine {
+    DevBuf state;        // LargeState + LargeInstance[]
+    DevBuf pool;         // bitmaps, rings, tables of the instances
+    hipGraphExec_t graph = nullptr;
+    int graph_levels = 0;
+    int graph_grid = 0;
+    void *graph_state = nullptr;
+    void release();
+    // Solves the listed automata of db (indices into db) concurrently.
+    int solve_indices(ssw_context &ctx, const DeviceBatch &db, const uint8_t *host_n,
+                      const std::vector<uint32_t> &idx);
+    int solve_all(ssw_context &ctx, const DeviceBatch &db, const uint8_t *host_n, int64_t *violation_out);
+};
+
+}  // namespace ssw
+
+struct ssw_context {
+    int device = -1;
+    int cus = 0;
+    size_t lds_per_cu = 0;
+    hipStream_t stream = nullptr;
+    hipStream_t monitor = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    uint64_t budget = 0;
+    std::mutex mu;           // serializes calls on the context
+    std::mutex monitor_mu;   // progress reads
+    std::string err;
+
+    ssw::DevBuf ctrs, deferred, order;
+    ssw::DevBuf d_delta, d_nper, d_off, d_len, d_sync, d_exp, d_wit;
+    std::map<ssw::SlotKey, ssw::DevBuf> slot_cache;
+    ssw::LargeEngine large;
+    ssw_tuning tune[SSW_MAX_N + 1];
+
+    double last_ms = 0;
+    int64_t last_launches = 0;
+    int64_t launches = 0;
+    uint64_t deferred_total = 0;
+    std::atomic<uint64_t> total_launches{0};
+    std::atomic<uint64_t> large_done{0};
+    std::atomic<uint64_t> large_subsets{0};
+    std::atomic<bool> solving{false};
+
+    // progress callback
+    std::thread prog_thread;
+    std::atomic<bool> prog_stop{false};
+    ssw_progress_fn prog_fn = nullptr;
+    void *prog_user = nullptr;
+    int prog_ms = 100;
+
+    int open(int dev, uint64_t mem_budget);
+    void close();
+    void default_tuning();
+    int fail(int code, const char *fmt, ...);
+    int hip_fail(hipError_t e, const char *what);
+    int ensure(ssw::DevBuf &b, size_t bytes, const char *what);
+    int get_slots(const ssw::SlotKey &key, uint64_t bytes, ssw::DevBuf **out);
+    int launch_class(const ssw::DeviceBatch &db, const ssw::ClassSpec &cls, bool plain);
+    int solve_device(const ssw::DeviceBatch &db, const uint8_t *host_n, bool plain, int64_t *violation_out);
+    int solve_host_packed(const ssw_batch *b, int mode, int64_t *violation_out);
+    int solve_entry(const ssw_batch *b, int32_t memory, int mode, int64_t *violation_out);
+    int compute_i32(int32_t n, int64_t count, const int32_t *A, const int32_t *B, int32_t *length_out,
+                    uint8_t *sync_out, uint8_t *witness_out, int32_t witness_stride, int64_t *expanded_out,
+                    int32_t bound, int64_t *violation_out);
+    int read_progress(ssw_progress *out);
+    void start_progress_thread();
+    void stop_progress_thread();
+};

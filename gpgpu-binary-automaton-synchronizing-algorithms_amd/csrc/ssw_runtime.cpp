@@ -1,0 +1,741 @@
+// ssw_runtime.cpp -- the C ABI (include/ssw.h): contexts, buffers, dispatch of
+// the batched / plain / device-wide solvers, tuning table, progress counters.
+//
+// The launcher replaces SlimGPUQueue.ComputeManyWithAction
+// (GPGPU/GPGPU/GPU/SlimGPUQueue.cs:32-208): instead of allocating and freeing
+// device arrays and streams on every call, a context keeps one stream, grows
+// its device buffers monotonically and keeps per-group scratch ("slots")
+// resident and clean between calls.
+#include "ssw.h"
+#include "ssw_internal.h"
+#include "ssw_host.h"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+namespace ssw {
+hipError_t launch_batch(int group_lanes, const BatchParams &P, int grid, int block, size_t lds,
+                        hipStream_t st);
+hipError_t batch_occupancy(int group_lanes, int block, size_t lds, int *blocks);
+size_t batch_scratch_bytes(int group_lanes);
+hipError_t fill_u32(uint32_t *p, uint64_t words, uint32_t value, hipStream_t st);
+}  // namespace ssw
+
+using namespace ssw;
+
+static thread_local std::string g_create_error;
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+
+int ssw_context::fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    err = buf;
+    return code;
+}
+
+int ssw_context::hip_fail(hipError_t e, const char *what) {
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
+        return fail(SSW_ERR_NOMEM, "%s: %s", what, hipGetErrorString(e));
+    return fail(SSW_ERR_DEVICE, "%s: %s", what, hipGetErrorString(e));
+}
+
+#define HIP_TRY(expr, what)                          \
+    do {                                             \
+        hipError_t _e = (expr);                      \
+        if (_e != hipSuccess) return hip_fail(_e, what); \
+    } while (0)
+
+int ssw_context::ensure(DevBuf &b, size_t bytes, const char *what) {
+    if (b.bytes >= bytes && b.p) return SSW_OK;
+    if (b.p) {
+        hipFree(b.p);
+        b.p = nullptr;
+        b.bytes = 0;
+    }
+    size_t want = bytes < 256 ? 256 : bytes;
+    hipError_t e = hipMalloc(&b.p, want);
+    if (e != hipSuccess) {
+        b.p = nullptr;
+        return hip_fail(e, what);
+    }
+    b.bytes = want;
+    return SSW_OK;
+}
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+
+int ssw_context::open(int dev, uint64_t mem_budget) {
+    device = dev;
+    HIP_TRY(hipSetDevice(dev), "hipSetDevice");
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
+    cus = prop.multiProcessorCount;
+    lds_per_cu = prop.maxSharedMemoryPerMultiProcessor;
+    HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+    HIP_TRY(hipStreamCreateWithFlags(&monitor, hipStreamNonBlocking), "hipStreamCreate");
+    HIP_TRY(hipEventCreate(&ev0), "hipEventCreate");
+    HIP_TRY(hipEventCreate(&ev1), "hipEventCreate");
+    size_t free_b = 0, total_b = 0;
+    HIP_TRY(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+    budget = mem_budget ? mem_budget : (uint64_t)(0.8 * (double)free_b);
+    int rc = ensure(ctrs, kCtrBytes, "counters");
+    if (rc) return rc;
+    HIP_TRY(hipMemsetAsync(ctrs.p, 0, kCtrBytes, stream), "hipMemset");
+    HIP_TRY(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    default_tuning();
+    return SSW_OK;
+}
+
+void ssw_context::close() {
+    stop_progress_thread();
+    if (device >= 0) hipSetDevice(device);
+    for (DevBuf *b : {&ctrs, &deferred, &order, &d_delta, &d_nper, &d_off, &d_len, &d_sync, &d_exp,
+                      &d_wit})
+        if (b->p) hipFree(b->p);
+    for (auto &kv : slot_cache)
+        if (kv.second.p) hipFree(kv.second.p);
+    slot_cache.clear();
+    large.release();
+    if (ev0) hipEventDestroy(ev0);
+    if (ev1) hipEventDestroy(ev1);
+    if (stream) hipStreamDestroy(stream);
+    if (monitor) hipStreamDestroy(monitor);
+}
+
+void ssw_context::default_tuning() {
+    for (int n = 0; n <= SSW_MAX_N; ++n) {
+        ssw_tuning t;
+        if (n <= 10) {
+            t.group_lanes = 16; t.block_threads = 256; t.chunk = 16; t.arena_lds = 256;
+        } else if (n <= 13) {
+            t.group_lanes = 32; t.block_threads = 256; t.chunk = 8; t.arena_lds = 512;
+        } else if (n <= 16) {
+            t.group_lanes = 64; t.block_threads = 256; t.chunk = 4; t.arena_lds = 1024;
+        } else {
+            t.group_lanes = 64; t.block_threads = 256; t.chunk = 2; t.arena_lds = 2048;
+        }
+        tune[n] = t;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// batched solve on device buffers
+// ---------------------------------------------------------------------------
+
+// Per-group scratch in HBM for one launch configuration.
+int ssw_context::get_slots(const SlotKey &key, uint64_t bytes, DevBuf **out) {
+    auto it = slot_cache.find(key);
+    if (it != slot_cache.end() && it->second.bytes >= bytes) {
+        *out = &it->second;
+        return SSW_OK;
+    }
+    if (it != slot_cache.end()) {
+        hipFree(it->second.p);
+        slot_cache.erase(it);
+    }
+    uint64_t used = 0;
+    for (auto &kv : slot_cache) used += kv.second.bytes;
+    // evict other configurations rather than exceed the budget
+    while (!slot_cache.empty() && used + bytes > budget) {
+        auto victim = slot_cache.begin();
+        used -= victim->second.bytes;
+        hipFree(victim->second.p);
+        slot_cache.erase(victim);
+    }
+    DevBuf b;
+    int rc = ensure(b, bytes, "slot scratch");
+    if (rc) return rc;
+    // bitmaps must start clean; the kernels keep them clean afterwards
+    HIP_TRY(hipMemsetAsync(b.p, 0, b.bytes, stream), "hipMemset slots");
+    slot_cache[key] = b;
+    *out = &slot_cache[key];
+    return SSW_OK;
+}
+
+static inline uint64_t align_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
+
+int ssw_context::launch_class(const DeviceBatch &db, const ClassSpec &cls, bool plain) {
+    const ssw_tuning t = plain ? ssw_tuning{256, 256, 1, 1024} : tune[cls.nmax];
+    const int G = t.group_lanes;
+    const int block = G == 256 ? 256 : std::max(64, std::min(256, t.block_threads));
+    const int groups_per_block = block / G;
+    const bool use_lds = cls.lds;
+    const int nmax = cls.nmax;
+    const bool want_wit = db.witness != nullptr;
+
+    BatchParams P;
+    std::memset(&P, 0, sizeof P);
+    P.count = plain ? db.count : (int64_t)cls.items;
+    P.uniform_n = db.uniform_n;
+    P.bound = db.bound;
+    P.n_per = db.n_per;
+    P.offset = db.offset;
+    P.delta = db.delta;
+    P.order = cls.order;
+    P.length = db.length;
+    P.sync = db.sync;
+    P.expanded = db.expanded;
+    P.witness = db.witness;
+    P.witness_stride = db.witness_stride;
+    P.want_witness = want_wit ? 1u : 0u;
+    uint8_t *c = (uint8_t *)ctrs.p;
+    P.work_counter = (unsigned int *)(c + kCtrWork);
+    P.deferred_count = (unsigned int *)(c + kCtrDeferred);
+    P.violation = (unsigned long long *)(c + kCtrViolation);
+    P.progress = (unsigned long long *)(c + kCtrProgress);
+    P.deferred_list = (uint32_t *)deferred.p;
+    P.chunk = (uint32_t)std::max(1, t.chunk);
+    P.lds_n = use_lds ? nmax : 0;
+    const uint32_t bm_words = use_lds ? (uint32_t)align_up(nmax >= 5 ? (1u << (nmax - 5)) : 1u, 4) : 4u;
+    P.lds_bitmap_words = bm_words;
+    uint32_t arena_lds = (uint32_t)align_up((uint64_t)std::max(4, t.arena_lds), 4);
+    // the LDS arena never needs more than every subset of Q
+    const uint64_t all_subsets = 1ull << nmax;
+    if (arena_lds > all_subsets + 2u * G) arena_lds = (uint32_t)align_up(all_subsets + 2u * G, 4);
+    P.lds_arena_cap = arena_lds;
+    const size_t per_group = kTableBytes + 4ull * bm_words + 4ull * arena_lds;
+    const size_t lds = per_group * groups_per_block + batch_scratch_bytes(G);
+    if (lds > 65536) return fail(SSW_ERR_ARG, "tuning for n=%d needs %zu bytes of LDS", nmax, lds);
+
+    // global arena: every subset fits when n <= 24 or the bitmap is in LDS;
+    // larger automata whose frontier outgrows it are deferred
+    uint64_t garena_cap;
+    if (use_lds || nmax <= 20)
+        garena_cap = all_subsets + 2u * G > arena_lds ? all_subsets + 2u * G - arena_lds : 0;
+    else
+        garena_cap = plain ? (1ull << 22) : (1ull << 18);
+    const uint64_t gbm_words = use_lds ? 0 : std::max<uint64_t>(1, all_subsets / 32);
+    const uint64_t good_words = want_wit ? std::max<uint64_t>(1, all_subsets / 32) : 0;
+    const uint32_t lvl_cap = want_wit ? (uint32_t)((nmax - 1) * (nmax - 1) + 8) : 0;
+
+    uint64_t off = 0;
+    P.garena_off = off;
+    P.garena_cap = garena_cap;
+    off = align_up(off + 4 * garena_cap, 256);
+    P.gbitmap_off = off;
+    P.gbitmap_words = gbm_words;
+    off = align_up(off + 4 * gbm_words, 256);
+    P.good_off = off;
+    P.good_words = good_words;
+    off = align_up(off + 8 * good_words, 256);
+    P.lvl_off = off;
+    P.lvl_cap = lvl_cap;
+    off = align_up(off + 4ull * lvl_cap, 256);
+    P.slot_bytes = std::max<uint64_t>(off, 256);
+
+    int grid = 1;
+    if (!plain) {
+        int per_cu = 0;
+        HIP_TRY(batch_occupancy(G, block, lds, &per_cu), "occupancy");
+        if (per_cu < 1) return fail(SSW_ERR_ARG, "kernel does not fit a CU (lds %zu)", lds);
+        grid = per_cu * cus;
+        const int64_t need = (P.count + groups_per_block - 1) / groups_per_block;
+        if (grid > need) grid = (int)std::max<int64_t>(1, need);
+        // stay inside the memory budget
+        const uint64_t max_slots = std::max<uint64_t>(1, budget / 2 / P.slot_bytes);
+        while ((uint64_t)grid * groups_per_block > max_slots && grid > 1) grid = grid / 2;
+    }
+    const uint64_t slots = (uint64_t)grid * groups_per_block;
+    SlotKey key{G, block, P.lds_n, nmax, (int)want_wit, (int)arena_lds, plain ? 1 : 0};
+    DevBuf *sb = nullptr;
+    int rc = get_slots(key, slots * P.slot_bytes, &sb);
+    if (rc) return rc;
+    P.slot_base = (uint8_t *)sb->p;
+
+    if (!plain) {
+        HIP_TRY(fill_u32((uint32_t *)P.work_counter, 1, 0, stream), "fill");
+        HIP_TRY(launch_batch(G, P, grid, block, lds, stream), "batch kernel launch");
+        ++launches;
+    } else {
+        for (int64_t i = 0; i < db.count; ++i) {
+            if (cls.host_n && cls.host_n[i] != cls.plain_n) continue;
+            P.single_item = 1;
+            P.first_item = i;
+            HIP_TRY(launch_batch(G, P, 1, block, lds, stream), "plain kernel launch");
+            ++launches;
+        }
+    }
+    return SSW_OK;
+}
+
+// Splits the batch into launch classes: (thread-group width, bitmap in LDS).
+int ssw_context::solve_device(const DeviceBatch &db, const uint8_t *host_n, bool plain,
+                              int64_t *violation_out) {
+    launches = 0;
+    int rc = ensure(deferred, 4ull * std::max<int64_t>(1, db.count), "deferred list");
+    if (rc) return rc;
+    // violation <- UINT64_MAX, deferred count <- 0
+    HIP_TRY(fill_u32((uint32_t *)((uint8_t *)ctrs.p + kCtrViolation), 2, 0xFFFFFFFFu, stream), "fill");
+    HIP_TRY(fill_u32((uint32_t *)((uint8_t *)ctrs.p + kCtrDeferred), 1, 0, stream), "fill");
+    HIP_TRY(hipEventRecord(ev0, stream), "hipEventRecord");
+
+    int nmin = db.uniform_n, nmaxall = db.uniform_n;
+    std::vector<int64_t> hist(SSW_MAX_N + 1, 0);
+    if (db.uniform_n > 0) {
+        hist[db.uniform_n] = db.count;
+    } else {
+        nmin = SSW_MAX_N;
+        nmaxall = 1;
+        for (int64_t i = 0; i < db.count; ++i) {
+            const int n = host_n[i];
+            if (n < 1 || n > SSW_MAX_N) return fail(SSW_ERR_ARG, "automaton %lld: n=%d out of [1,30]", (long long)i, n);
+            hist[n]++;
+            nmin = std::min(nmin, n);
+            nmaxall = std::max(nmaxall, n);
+        }
+    }
+
+    if (plain) {
+        // one launch per automaton, with the LDS bitmap whenever n <= 16
+        for (int n = 1; n <= SSW_MAX_N; ++n) {
+            if (!hist[n]) continue;
+            ClassSpec cls;
+            cls.nmax = n;
+            cls.lds = n <= kMaxLdsN;
+            cls.items = hist[n];
+            cls.order = nullptr;
+            cls.host_n = db.uniform_n > 0 ? nullptr : host_n;
+            cls.plain_n = n;
+            rc = launch_class(db, cls, true);
+            if (rc) return rc;
+        }
+    } else {
+        // class key: group lanes and LDS-or-HBM bitmap; automata of a class
+        // are ordered largest n first so the long ones start early
+        std::map<std::pair<int, int>, std::vector<int>> classes;  // (G, lds) -> list of n
+        for (int n = 1; n <= SSW_MAX_N; ++n)
+            if (hist[n]) classes[{tune[n].group_lanes, n <= kMaxLdsN ? 1 : 0}].push_back(n);
+        const bool single_class = classes.size() == 1 && db.uniform_n > 0;
+        std::vector<uint32_t> host_order;
+        if (!single_class) {
+            rc = ensure(order, 4ull * db.count, "order");
+            if (rc) return rc;
+            host_order.reserve(db.count);
+        }
+        for (auto &kv : classes) {
+            ClassSpec cls;
+            cls.lds = kv.first.second != 0;
+            cls.nmax = *std::max_element(kv.second.begin(), kv.second.end());
+            cls.items = 0;
+            for (int n : kv.second) cls.items += hist[n];
+            cls.order = nullptr;
+            cls.host_n = nullptr;
+            cls.plain_n = 0;
+            if (!single_class) {
+                const size_t start = host_order.size();
+                std::vector<int> ns = kv.second;
+                std::sort(ns.rbegin(), ns.rend());
+                for (int n : ns)
+                    for (int64_t i = 0; i < db.count; ++i)
+                        if ((db.uniform_n > 0 ? db.uniform_n : host_n[i]) == n) host_order.push_back((uint32_t)i);
+                cls.order = (const uint32_t *)order.p + start;
+                HIP_TRY(hipMemcpyAsync((uint32_t *)order.p + start, host_order.data() + start,
+                                       4ull * (host_order.size() - start), hipMemcpyHostToDevice, stream),
+                        "upload order");
+            }
+            rc = launch_class(db, cls, false);
+            if (rc) return rc;
+        }
+    }
+    HIP_TRY(hipEventRecord(ev1, stream), "hipEventRecord");
+
+    // automata whose frontier outgrew a group's arena: device-wide path
+    uint32_t ndef = 0;
+    HIP_TRY(hipMemcpyAsync(&ndef, (uint8_t *)ctrs.p + kCtrDeferred, 4, hipMemcpyDeviceToHost, stream), "D2H");
+    HIP_TRY(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1), "hipEventElapsedTime");
+    last_ms = ms;
+    last_launches = launches;
+    if (ndef) {
+        std::vector<uint32_t> idx(ndef);
+        HIP_TRY(hipMemcpy(idx.data(), deferred.p, 4ull * ndef, hipMemcpyDeviceToHost), "D2H deferred");
+        deferred_total += ndef;
+        rc = large.solve_indices(*this, db, host_n, idx);
+        if (rc) return rc;
+    }
+    unsigned long long viol = ~0ull;
+    HIP_TRY(hipMemcpyAsync(&viol, (uint8_t *)ctrs.p + kCtrViolation, 8, hipMemcpyDeviceToHost, stream), "D2H");
+    HIP_TRY(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    if (violation_out) *violation_out = viol == ~0ull ? -1 : (int64_t)viol;
+    return SSW_OK;
+}
+
+// ---------------------------------------------------------------------------
+// host-buffer front ends
+// ---------------------------------------------------------------------------
+
+int ssw_context::solve_host_packed(const ssw_batch *b, int mode, int64_t *violation_out) {
+    const int64_t count = b->count;
+    // extent of the packed transitions and validation
+    uint64_t extent = 0;
+    for (int64_t i = 0; i < count; ++i) {
+        const int n = b->uniform_n > 0 ? b->uniform_n : b->n_per[i];
+        if (n < 1 || n > SSW_MAX_N) return fail(SSW_ERR_ARG, "automaton %lld: n=%d out of [1,30]", (long long)i, n);
+        const uint64_t o = b->offset ? b->offset[i] : (uint64_t)i * 2u * (uint64_t)n;
+        const uint8_t *d = b->delta + o;
+        for (int q = 0; q < 2 * n; ++q)
+            if (d[q] >= n) return fail(SSW_ERR_ARG, "automaton %lld: transition %d >= n", (long long)i, d[q]);
+        extent = std::max<uint64_t>(extent, o + 2u * (uint64_t)n);
+    }
+    int rc;
+    if ((rc = ensure(d_delta, extent, "delta"))) return rc;
+    if ((rc = ensure(d_len, 4ull * count, "length"))) return rc;
+    HIP_TRY(hipMemcpyAsync(d_delta.p, b->delta, extent, hipMemcpyHostToDevice, stream), "H2D");
+    DeviceBatch db;
+    db.count = count;
+    db.uniform_n = b->uniform_n;
+    db.bound = b->bound;
+    db.delta = (const uint8_t *)d_delta.p;
+    db.length = (int32_t *)d_len.p;
+    db.witness_stride = b->witness_stride;
+    if (b->uniform_n <= 0) {
+        if ((rc = ensure(d_nper, count, "n_per"))) return rc;
+        HIP_TRY(hipMemcpyAsync(d_nper.p, b->n_per, count, hipMemcpyHostToDevice, stream), "H2D");
+        db.n_per = (const uint8_t *)d_nper.p;
+    }
+    if (b->offset) {
+        if ((rc = ensure(d_off, 8ull * count, "offset"))) return rc;
+        HIP_TRY(hipMemcpyAsync(d_off.p, b->offset, 8ull * count, hipMemcpyHostToDevice, stream), "H2D");
+        db.offset = (const uint64_t *)d_off.p;
+    }
+    if (b->sync) {
+        if ((rc = ensure(d_sync, count, "sync"))) return rc;
+        db.sync = (uint8_t *)d_sync.p;
+    }
+    if (b->expanded) {
+        if ((rc = ensure(d_exp, 8ull * count, "expanded"))) return rc;
+        db.expanded = (int64_t *)d_exp.p;
+    }
+    if (b->witness) {
+        if ((rc = ensure(d_wit, (uint64_t)b->witness_stride * count, "witness"))) return rc;
+        db.witness = (uint8_t *)d_wit.p;
+    }
+    rc = mode == kModeLarge ? large.solve_all(*this, db, b->uniform_n > 0 ? nullptr : b->n_per, violation_out)
+                            : solve_device(db, b->uniform_n > 0 ? nullptr : b->n_per, mode == kModePlain,
+                                           violation_out);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(b->length, db.length, 4ull * count, hipMemcpyDeviceToHost, stream), "D2H");
+    if (b->sync) HIP_TRY(hipMemcpyAsync(b->sync, db.sync, count, hipMemcpyDeviceToHost, stream), "D2H");
+    if (b->expanded)
+        HIP_TRY(hipMemcpyAsync(b->expanded, db.expanded, 8ull * count, hipMemcpyDeviceToHost, stream), "D2H");
+    if (b->witness)
+        HIP_TRY(hipMemcpyAsync(b->witness, db.witness, (uint64_t)b->witness_stride * count,
+                               hipMemcpyDeviceToHost, stream),
+                "D2H");
+    HIP_TRY(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    return SSW_OK;
+}
+
+int ssw_context::solve_entry(const ssw_batch *b, int32_t memory, int mode, int64_t *violation_out) {
+    if (!b || b->count < 0 || !b->delta || !b->length) return fail(SSW_ERR_ARG, "null batch field");
+    if (b->count > (int64_t)0x7FFFFFFF) return fail(SSW_ERR_ARG, "count above 2^31-1");
+    if (b->uniform_n <= 0 && !b->n_per) return fail(SSW_ERR_ARG, "n_per is NULL and uniform_n <= 0");
+    if (b->uniform_n > SSW_MAX_N) return fail(SSW_ERR_ARG, "n=%d above %d", b->uniform_n, SSW_MAX_N);
+    if (b->witness) {
+        int nmax = b->uniform_n;
+        if (nmax <= 0) {
+            nmax = 1;
+            if (memory == SSW_HOST)
+                for (int64_t i = 0; i < b->count; ++i) nmax = std::max<int>(nmax, b->n_per[i]);
+            else
+                nmax = SSW_MAX_N;
+        }
+        const int need = ((nmax - 1) * (nmax - 1) + 7) / 8;
+        if (b->witness_stride < std::max(1, need))
+            return fail(SSW_ERR_ARG, "witness_stride %d < %d bytes", b->witness_stride, need);
+    }
+    if (violation_out) *violation_out = -1;
+    if (b->count == 0) return SSW_OK;
+    HIP_TRY(hipSetDevice(device), "hipSetDevice");
+    solving.store(true);
+    int rc;
+    if (memory == SSW_HOST) {
+        rc = solve_host_packed(b, mode, violation_out);
+    } else if (memory == SSW_DEVICE) {
+        DeviceBatch db;
+        db.count = b->count;
+        db.uniform_n = b->uniform_n;
+        db.bound = b->bound;
+        db.n_per = b->n_per;
+        db.offset = b->offset;
+        db.delta = b->delta;
+        db.length = b->length;
+        db.sync = b->sync;
+        db.expanded = b->expanded;
+        db.witness = b->witness;
+        db.witness_stride = b->witness_stride;
+        std::vector<uint8_t> host_n;
+        if (b->uniform_n <= 0) {
+            host_n.resize(b->count);
+            hipError_t e = hipMemcpy(host_n.data(), b->n_per, b->count, hipMemcpyDeviceToHost);
+            if (e != hipSuccess) {
+                solving.store(false);
+                return hip_fail(e, "D2H n_per");
+            }
+        }
+        const uint8_t *hn = b->uniform_n > 0 ? nullptr : host_n.data();
+        rc = mode == kModeLarge ? large.solve_all(*this, db, hn, violation_out)
+                                : solve_device(db, hn, mode == kModePlain, violation_out);
+    } else {
+        rc = fail(SSW_ERR_ARG, "memory kind %d", memory);
+    }
+    solving.store(false);
+    return rc;
+}
+
+int ssw_context::compute_i32(int32_t n, int64_t count, const int32_t *A, const int32_t *B,
+                             int32_t *length_out, uint8_t *sync_out, uint8_t *witness_out,
+                             int32_t witness_stride, int64_t *expanded_out, int32_t bound,
+                             int64_t *violation_out) {
+    if (n < 1 || n > SSW_MAX_N) return fail(SSW_ERR_ARG, "n=%d out of [1,30]", n);
+    if (count < 0 || (count > 0 && (!A || !B || !length_out))) return fail(SSW_ERR_ARG, "null argument");
+    // flatten like SlimGPUQueue.cs:108-126, packed to bytes
+    std::vector<uint8_t> packed((size_t)count * 2 * n);
+    for (int64_t i = 0; i < count; ++i) {
+        uint8_t *d = packed.data() + (size_t)i * 2 * n;
+        for (int q = 0; q < n; ++q) {
+            const int32_t x = A[i * n + q], y = B[i * n + q];
+            if (x < 0 || x >= n || y < 0 || y >= n)
+                return fail(SSW_ERR_ARG, "automaton %lld state %d: transition out of [0,%d)", (long long)i, q, n);
+            d[q] = (uint8_t)x;
+            d[n + q] = (uint8_t)y;
+        }
+    }
+    ssw_batch b;
+    std::memset(&b, 0, sizeof b);
+    b.count = count;
+    b.uniform_n = n;
+    b.delta = packed.data();
+    b.length = length_out;
+    b.sync = sync_out;
+    b.expanded = expanded_out;
+    b.witness = witness_out;
+    b.witness_stride = witness_stride;
+    b.bound = bound;
+    return solve_entry(&b, SSW_HOST, kModeBatch, violation_out);
+}
+
+// ---------------------------------------------------------------------------
+// progress
+// ---------------------------------------------------------------------------
+
+int ssw_context::read_progress(ssw_progress *out) {
+    std::lock_guard<std::mutex> lk(monitor_mu);
+    unsigned long long v[2] = {0, 0};
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(v, (uint8_t *)ctrs.p + kCtrProgress, sizeof v, hipMemcpyDeviceToHost, monitor);
+    if (e == hipSuccess) e = hipStreamSynchronize(monitor);
+    if (e != hipSuccess) return SSW_ERR_DEVICE;
+    out->automata_done = v[0] + large_done.load();
+    out->subsets_expanded = v[1] + large_subsets.load();
+    out->launches = total_launches.load();
+    out->deferred = deferred_total;
+    return SSW_OK;
+}
+
+void ssw_context::start_progress_thread() {
+    stop_progress_thread();
+    prog_stop.store(false);
+    prog_thread = std::thread([this] {
+        bool was_solving = false;
+        while (!prog_stop.load()) {
+            std::this_thread::sleep_for(std::chrono::milliseconds(prog_ms));
+            const bool now = solving.load();
+            if (now || was_solving) {
+                ssw_progress p;
+                if (read_progress(&p) == SSW_OK && prog_fn) prog_fn(&p, prog_user);
+            }
+            was_solving = now;
+        }
+    });
+}
+
+void ssw_context::stop_progress_thread() {
+    if (prog_thread.joinable()) {
+        prog_stop.store(true);
+        prog_thread.join();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// extern "C"
+// ---------------------------------------------------------------------------
+
+#define CTX_GUARD(ctx)                                           \
+    if (!(ctx)) return SSW_ERR_ARG;                              \
+    std::lock_guard<std::mutex> _lk((ctx)->mu)
+
+extern "C" {
+
+int ssw_abi_version(void) { return SSW_ABI_VERSION; }
+
+int ssw_create(int device, uint64_t mem_budget, uint32_t flags, ssw_context **out) {
+    (void)flags;
+    if (!out) return SSW_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        g_create_error = "no HIP device";
+        return SSW_ERR_DEVICE;
+    }
+    if (device < 0 || device >= ndev) {
+        g_create_error = "device ordinal out of range";
+        return SSW_ERR_ARG;
+    }
+    ssw_context *c = new (std::nothrow) ssw_context();
+    if (!c) return SSW_ERR_NOMEM;
+    int rc = c->open(device, mem_budget);
+    if (rc) {
+        g_create_error = c->err;
+        c->close();
+        delete c;
+        return rc;
+    }
+    *out = c;
+    return SSW_OK;
+}
+
+int ssw_destroy(ssw_context *ctx) {
+    if (!ctx) return SSW_ERR_ARG;
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        ctx->close();
+    }
+    delete ctx;
+    return SSW_OK;
+}
+
+const char *ssw_last_error(const ssw_context *ctx) {
+    return ctx ? ctx->err.c_str() : g_create_error.c_str();
+}
+
+int ssw_compute(ssw_context *ctx, int32_t n, int64_t count, const int32_t *delta_a,
+                const int32_t *delta_b, int32_t *length_out, uint8_t *sync_out, uint8_t *witness_out,
+                int32_t witness_stride, int64_t *expanded_out, int32_t bound, int64_t *violation_out) {
+    CTX_GUARD(ctx);
+    return ctx->compute_i32(n, count, delta_a, delta_b, length_out, sync_out, witness_out, witness_stride,
+                            expanded_out, bound, violation_out);
+}
+
+int ssw_verify(ssw_context *ctx, int32_t n, int64_t count, const int32_t *delta_a, const int32_t *delta_b,
+               int32_t bound, int64_t *violation_out) {
+    CTX_GUARD(ctx);
+    std::vector<int32_t> len((size_t)std::max<int64_t>(count, 0));
+    return ctx->compute_i32(n, count, delta_a, delta_b, len.data(), nullptr, nullptr, 0, nullptr, bound,
+                            violation_out);
+}
+
+int ssw_best_parallelism(ssw_context *ctx) { return ctx ? 1 : SSW_ERR_ARG; }
+
+int ssw_solve_batch(ssw_context *ctx, const ssw_batch *b, int32_t memory, int64_t *violation_out) {
+    CTX_GUARD(ctx);
+    return ctx->solve_entry(b, memory, kModeBatch, violation_out);
+}
+
+int ssw_solve_plain(ssw_context *ctx, const ssw_batch *b, int32_t memory, int64_t *violation_out) {
+    CTX_GUARD(ctx);
+    return ctx->solve_entry(b, memory, kModePlain, violation_out);
+}
+
+int ssw_solve_large(ssw_context *ctx, const ssw_batch *b, int32_t memory, int64_t *violation_out) {
+    CTX_GUARD(ctx);
+    return ctx->solve_entry(b, memory, kModeLarge, violation_out);
+}
+
+int ssw_get_tuning(ssw_context *ctx, int32_t n, ssw_tuning *out) {
+    CTX_GUARD(ctx);
+    if (n < 1 || n > SSW_MAX_N || !out) return ctx->fail(SSW_ERR_ARG, "bad n or null");
+    *out = ctx->tune[n];
+    return SSW_OK;
+}
+
+int ssw_set_tuning(ssw_context *ctx, int32_t n, const ssw_tuning *t) {
+    CTX_GUARD(ctx);
+    if (n < 1 || n > SSW_MAX_N || !t) return ctx->fail(SSW_ERR_ARG, "bad n or null");
+    const int G = t->group_lanes;
+    if (G != 16 && G != 32 && G != 64 && G != 256) return ctx->fail(SSW_ERR_ARG, "group_lanes %d", G);
+    if (t->block_threads % 64 || t->block_threads < 64 || t->block_threads > 256 ||
+        (G != 256 && t->block_threads % G) || (G == 256 && t->block_threads != 256))
+        return ctx->fail(SSW_ERR_ARG, "block_threads %d", t->block_threads);
+    if (t->chunk < 1 || t->arena_lds < 4) return ctx->fail(SSW_ERR_ARG, "chunk/arena_lds");
+    ctx->tune[n] = *t;
+    return SSW_OK;
+}
+
+int ssw_get_progress(ssw_context *ctx, ssw_progress *out) {
+    if (!ctx || !out) return SSW_ERR_ARG;
+    return ctx->read_progress(out);
+}
+
+int ssw_set_progress_callback(ssw_context *ctx, ssw_progress_fn fn, void *user, int32_t interval_ms) {
+    if (!ctx) return SSW_ERR_ARG;
+    ctx->stop_progress_thread();
+    ctx->prog_fn = fn;
+    ctx->prog_user = user;
+    ctx->prog_ms = interval_ms > 0 ? interval_ms : 100;
+    if (fn) ctx->start_progress_thread();
+    return SSW_OK;
+}
+
+int ssw_device_alloc(ssw_context *ctx, uint64_t bytes, void **out) {
+    if (!ctx || !out) return SSW_ERR_ARG;
+    hipSetDevice(ctx->device);
+    hipError_t e = hipMalloc(out, bytes ? bytes : 1);
+    return e == hipSuccess ? SSW_OK : (e == hipErrorOutOfMemory ? SSW_ERR_NOMEM : SSW_ERR_DEVICE);
+}
+
+int ssw_device_free(ssw_context *ctx, void *p) {
+    if (!ctx) return SSW_ERR_ARG;
+    hipSetDevice(ctx->device);
+    return hipFree(p) == hipSuccess ? SSW_OK : SSW_ERR_DEVICE;
+}
+
+int ssw_copy_to_device(ssw_context *ctx, void *dst, const void *src, uint64_t bytes) {
+    CTX_GUARD(ctx);
+    hipSetDevice(ctx->device);
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    return e == hipSuccess ? SSW_OK : ctx->hip_fail(e, "copy to device");
+}
+
+int ssw_copy_to_host(ssw_context *ctx, void *dst, const void *src, uint64_t bytes) {
+    CTX_GUARD(ctx);
+    hipSetDevice(ctx->device);
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    return e == hipSuccess ? SSW_OK : ctx->hip_fail(e, "copy to host");
+}
+
+int ssw_synchronize(ssw_context *ctx) {
+    CTX_GUARD(ctx);
+    hipSetDevice(ctx->device);
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    return e == hipSuccess ? SSW_OK : ctx->hip_fail(e, "synchronize");
+}
+
+int ssw_last_kernel_time(ssw_context *ctx, double *ms, int64_t *launches) {
+    CTX_GUARD(ctx);
+    if (ms) *ms = ctx->last_ms;
+    if (launches) *launches = ctx->last_launches;
+    return SSW_OK;
+}
+
+}  // extern "C"

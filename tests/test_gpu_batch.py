@@ -1,0 +1,132 @@
+"""GPU parity of the batched persistent kernel and the plain one-per-launch path
+against the CPU oracle (restatement of SlimCPUUntweaked.cs:51-222 / CPU.cs:93-283).
+
+Bar: lengths, synchronizing flags and canonical expanded counts bit-exact;
+witnesses equal to the oracle's (the lexicographically smallest shortest reset
+word) and re-verified on Q as Verify.cs:12-53 does.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import ssw
+import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _check(n, A, B, res, witness_rows=None, tag=""):
+    L, S, E, _ = O.solve(n, A, B, threads=8)
+    bad = np.nonzero(res.length != L)[0]
+    assert len(bad) == 0, f"{tag}: length mismatch at {bad[:10]} gpu={res.length[bad[:10]]} cpu={L[bad[:10]]}"
+    np.testing.assert_array_equal(res.sync, S, err_msg=f"{tag}: sync")
+    np.testing.assert_array_equal(res.expanded, E, err_msg=f"{tag}: expanded")
+    if witness_rows is not None:
+        for i in witness_rows:
+            if L[i] < 0:
+                continue
+            w = ssw.witness_bits_to_word(res.witness[i], int(L[i]))
+            assert O.verify_word(n, A[i], B[i], w), f"{tag}: witness {i} does not synchronize"
+            if n <= 24:
+                Lw, wo = O.witness(n, A[i], B[i])
+                assert Lw == L[i]
+                np.testing.assert_array_equal(w, wo, err_msg=f"{tag}: witness {i} differs from oracle")
+
+
+@pytest.mark.parametrize("conv", [0, 1])
+def test_cerny_known_answers(ctx, conv):
+    # SimpleAutomataTests.cs:14-35 / IntegrityTest.cs:44-62: C_n has length (n-1)^2
+    for n in range(2, 17):
+        a, b = ssw.generate_cerny(n, conv)
+        A = np.tile(a, (3, 1))
+        B = np.tile(b, (3, 1))
+        r = ctx.compute(n, A, B, witness=True)
+        assert list(r.length) == [(n - 1) ** 2] * 3
+        assert list(r.expanded) == [2 ** n - n - 1] * 3
+        _check(n, A, B, r, witness_rows=range(3), tag=f"cerny{n}")
+        # closed form of the unique shortest word (b a^(n-1))^(n-2) b, letters swapped for conv 0
+        word = ([1] + [0] * (n - 1)) * (n - 2) + [1]
+        if conv == 0:
+            word = [1 - x for x in word]
+        np.testing.assert_array_equal(ssw.witness_bits_to_word(r.witness[0], (n - 1) ** 2), word)
+
+
+def test_plain_n10_full_with_witness(ctx):
+    n, A, B = W.plain_n10()
+    r = ctx.compute(n, A, B, witness=True)
+    _check(n, A, B, r, witness_rows=range(len(A)), tag="n10")
+
+
+def test_uniform_n16_slice(ctx):
+    n, delta, pos = W.uniform_n16(count=20_000)
+    A = delta[:, :n].astype(np.int32)
+    B = delta[:, n:].astype(np.int32)
+    r = ctx.solve_packed(delta.reshape(-1), len(delta), n)
+    _check(n, A, B, r, tag="n16")
+    assert np.all(r.length[pos] == 225)
+
+
+def test_uniform_n16_witness(ctx):
+    n, delta, pos = W.uniform_n16(count=3_000)
+    A = delta[:, :n].astype(np.int32)
+    B = delta[:, n:].astype(np.int32)
+    r = ctx.solve_packed(delta.reshape(-1), len(delta), n, witness=True)
+    _check(n, A, B, r, witness_rows=range(len(A)), tag="n16w")
+
+
+def test_mixed_sizes_small(ctx):
+    rng = np.random.default_rng(5)
+    ns = rng.integers(1, 17, size=3000).astype(np.uint8)
+    As, Bs = [], []
+    for i, n in enumerate(ns):
+        a, b = ssw.generate_uniform(77, int(n), 1, first=i)
+        As.append(a[0])
+        Bs.append(b[0])
+    r = ctx.solve_mixed(ns, As, Bs, witness=True)
+    for n in np.unique(ns):
+        idx = np.nonzero(ns == n)[0]
+        A = np.stack([As[i] for i in idx])
+        B = np.stack([Bs[i] for i in idx])
+        sub = ssw.Solved(r.length[idx], r.sync[idx], r.expanded[idx], r.witness[idx], -1)
+        _check(int(n), A, B, sub, witness_rows=range(len(idx)), tag=f"mixed n={n}")
+
+
+@pytest.mark.parametrize("n", [17, 18, 20])
+def test_hbm_bitmap(ctx, n):
+    A, B = ssw.generate_uniform(900 + n, n, 2000)
+    r = ctx.compute(n, A, B, witness=True)
+    _check(n, A, B, r, witness_rows=range(200), tag=f"hbm n={n}")
+
+
+def test_plain_path(ctx):
+    n, delta, pos = W.uniform_n16(count=300, cerny_every=100)
+    A = delta[:, :n].astype(np.int32)
+    B = delta[:, n:].astype(np.int32)
+    r = ctx.solve_packed(delta.reshape(-1), len(delta), n, witness=True, mode="plain")
+    _check(n, A, B, r, witness_rows=range(len(A)), tag="plain")
+
+
+def test_violation_index(ctx):
+    # a lowered bound exercises the Cerny-violation path (SlimCPUUntweaked.cs:189-213)
+    n, A, B = W.plain_n10(count=500)
+    L, _, _, v = O.slim(n, A, B, bound=7)
+    r = ctx.compute(n, A, B, bound=7)
+    assert r.violation == v == int(np.nonzero(L > 7)[0][0])
+    assert ctx.verify(n, A, B, bound=7) == v
+    assert ctx.verify(n, A, B) == -1
+
+
+def test_bad_transition_rejected(ctx):
+    A = np.zeros((2, 5), np.int32)
+    B = np.zeros((2, 5), np.int32)
+    B[1, 3] = 5
+    with pytest.raises(ssw.SswError):
+        ctx.compute(5, A, B)
+
+
+def test_repeatable(ctx):
+    n, delta, pos = W.uniform_n16(count=5_000)
+    r1 = ctx.solve_packed(delta.reshape(-1), len(delta), n, witness=True)
+    r2 = ctx.solve_packed(delta.reshape(-1), len(delta), n, witness=True)
+    np.testing.assert_array_equal(r1.length, r2.length)
+    np.testing.assert_array_equal(r1.witness, r2.witness)
