@@ -35,7 +35,10 @@ typedef __attribute__((address_space(3))) uint32_t lds_u32;
 typedef __attribute__((address_space(3))) uint64_t lds_u64;
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(3))) u32x4 lds_u4;
+typedef __attribute__((address_space(3))) uint16_t lds_u16;
+typedef __attribute__((address_space(3))) uint8_t lds_u8;
 typedef __attribute__((address_space(1))) uint32_t glb_u32;
+typedef __attribute__((address_space(1))) uint16_t glb_u16;
 typedef __attribute__((address_space(1))) uint8_t glb_u8;
 
 template <class T>
@@ -44,6 +47,25 @@ __device__ __forceinline__ uint32_t or_wg(T *p, uint32_t v) {
 }
 
 __device__ __forceinline__ bool is_single(uint32_t x) { return (x & (x - 1u)) == 0u; }
+
+// Marks a value as uniform across the wavefront (valid when a thread group is
+// a whole wavefront or workgroup), so the compiler keeps it in an SGPR and
+// branches on it with scalar branches instead of exec masks.
+template <int G>
+__device__ __forceinline__ uint32_t uni(uint32_t v) {
+    return G >= 64 ? __builtin_amdgcn_readfirstlane(v) : v;
+}
+template <int G>
+__device__ __forceinline__ int32_t uni(int32_t v) {
+    return G >= 64 ? (int32_t)__builtin_amdgcn_readfirstlane((uint32_t)v) : v;
+}
+template <int G>
+__device__ __forceinline__ int64_t uni(int64_t v) {
+    if (G < 64) return v;
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)(uint64_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)v >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
 
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
@@ -77,7 +99,26 @@ struct WaveGroup {
         off = __popcll(ba & below) + __popcll(bb & below);
         total = __popcll(ba) + __popcll(bb);
     }
-    __device__ uint32_t bcast0(uint32_t v) const { return __shfl(v, (int)gbase, 64); }
+    // any(single) and the exclusive prefix / total of per-lane counts
+    // c < 2^BITS, from one ballot per bit of c
+    template <int BITS>
+    __device__ void exchange_count(bool single, uint32_t c, bool &any_single, uint32_t &prefix,
+                                   uint32_t &total) const {
+        any_single = (__ballot(single) & gmask) != 0;
+        uint32_t pre = 0, tot = 0;
+#pragma unroll
+        for (int j = 0; j < BITS; ++j) {
+            const uint64_t bj = __ballot((c >> j) & 1u) & gmask;
+            pre += (uint32_t)__popcll(bj & below) << j;
+            tot += (uint32_t)__popcll(bj) << j;
+        }
+        prefix = pre;
+        total = tot;
+    }
+    __device__ uint32_t bcast0(uint32_t v) const {
+        if (G == 64) return __builtin_amdgcn_readfirstlane(v);
+        return __shfl(v, (int)gbase, 64);
+    }
 };
 
 // A whole workgroup of NT threads (one group per block).
@@ -130,13 +171,100 @@ struct BlockGroup {
         total = tot;
         any_single = anys;
     }
+    template <int BITS>
+    __device__ void exchange_count(bool single, uint32_t c, bool &any_single, uint32_t &prefix,
+                                   uint32_t &total) const {
+        const uint64_t bs = __ballot(single);
+        const uint64_t below = (1ull << lane) - 1ull;
+        uint32_t pre = 0, tot = 0;
+#pragma unroll
+        for (int j = 0; j < BITS; ++j) {
+            const uint64_t bj = __ballot((c >> j) & 1u);
+            pre += (uint32_t)__popcll(bj & below) << j;
+            tot += (uint32_t)__popcll(bj) << j;
+        }
+        __syncthreads();
+        if (lane == 0) {
+            scratch[wave] = tot;
+            scratch[kWaves + wave] = bs != 0;
+        }
+        __syncthreads();
+        uint32_t wpre = 0, all = 0;
+        bool anys = false;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+            const uint32_t x = scratch[w];
+            wpre += (w < (int)wave) ? x : 0u;
+            all += x;
+            anys |= scratch[kWaves + w] != 0;
+        }
+        prefix = wpre + pre;
+        total = all;
+        any_single = anys;
+    }
     __device__ uint32_t bcast0(uint32_t v) const {
         __syncthreads();
         if (tid == 0) scratch[2 * kWaves] = v;
         __syncthreads();
-        return scratch[2 * kWaves];
+        return __builtin_amdgcn_readfirstlane(scratch[2 * kWaves]);
     }
 };
+
+// Device work queue: the batch is cut into kQueues contiguous segments, each
+// with its own counter on its own 128-byte line, so the atomics of thousands
+// of groups spread over many addresses; a group starts at segment
+// (slot mod kQueues) and moves on when its segment runs dry, recording the
+// dry segment in a shared mask that the others skip.
+template <class Grp>
+__device__ int64_t fetch_work(const BatchParams &P, const Grp &g, uint32_t &q, int64_t &it, int64_t &it_end) {
+    if (it < it_end) return it++;
+    uint32_t base = 0xFFFFFFFFu, end = 0;
+    if (g.rank() == 0) {
+        unsigned long long *mask = (unsigned long long *)(P.queues + kQueues * kQueueWords);
+        for (int tries = 0; tries < kQueues; ++tries) {
+            const unsigned long long m = __hip_atomic_load(mask, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (m == ~0ull) break;
+            if ((m >> q) & 1ull) {
+                // next live segment after q (cyclically)
+                const unsigned long long live = ~m;
+                const unsigned long long hi_part = q + 1 < 64 ? (live >> (q + 1)) << (q + 1) : 0ull;
+                q = hi_part ? (uint32_t)__ffsll((long long)hi_part) - 1u : (uint32_t)__ffsll((long long)live) - 1u;
+            }
+            const uint32_t b = atomicAdd(P.queues + q * kQueueWords, P.chunk);
+            const uint32_t e = P.queues[q * kQueueWords + 1];
+            if (b < e) {
+                base = b;
+                end = b + P.chunk < e ? b + P.chunk : e;
+                break;
+            }
+            atomicOr(mask, 1ull << q);
+            q = (q + 1) & (kQueues - 1);
+        }
+    }
+    base = g.bcast0(base);
+    end = g.bcast0(end);
+    if (base == 0xFFFFFFFFu) return -1;
+    it = base;
+    it_end = end;
+    return it++;
+}
+
+// Sets up the segments of a batch of `count` work items.
+__global__ void init_queues_kernel(uint32_t *queues, int64_t count) {
+    const int k = threadIdx.x;
+    if (k < kQueues) {
+        const uint32_t s = (uint32_t)(count * k / kQueues), e = (uint32_t)(count * (k + 1) / kQueues);
+        queues[k * kQueueWords] = s;
+        queues[k * kQueueWords + 1] = e;
+    }
+    __syncthreads();
+    if (k == 0) {
+        unsigned long long m = 0;
+        for (int j = 0; j < kQueues; ++j)
+            if (queues[j * kQueueWords] >= queues[j * kQueueWords + 1]) m |= 1ull << j;
+        *(unsigned long long *)(queues + kQueues * kQueueWords) = m;
+    }
+}
 
 template <class Grp>
 struct GroupSolver {
@@ -161,6 +289,7 @@ struct GroupSolver {
     uint32_t lo, hi, wr, rd, level;
     // work queue
     int64_t it, it_end;
+    uint32_t wq;
     bool taken;
     // progress
     uint32_t done_local;
@@ -177,6 +306,7 @@ struct GroupSolver {
         lvl = (glb_u32 *)(slot + P.lvl_off);
         it = 0;
         it_end = 0;
+        wq = 0;
         taken = false;
         done_local = 0;
         subsets_local = 0;
@@ -215,16 +345,19 @@ struct GroupSolver {
             taken = true;
             return P.first_item;
         }
-        if (it >= it_end) {
-            uint32_t base = 0;
-            if (g.rank() == 0) base = atomicAdd(P.work_counter, P.chunk);
-            base = g.bcast0(base);
-            it = base;
-            it_end = (int64_t)base + P.chunk;
-            if (it_end > P.count) it_end = P.count;
-            if (it >= P.count) return -1;
-        }
-        return it++;
+        return fetch_work(P, g, wq, it, it_end);
+    }
+
+    int64_t nx_ai = -1;
+    __device__ void prefetch() {
+        const int64_t item = next_item();
+        nx_ai = item < 0 ? -1 : (P.order ? (int64_t)P.order[item] : item);
+    }
+    __device__ bool has_next() const { return nx_ai >= 0; }
+    __device__ bool start() {
+        const int64_t a = nx_ai;
+        prefetch();
+        return init(a);
     }
 
     // Loads automaton `a`; false when its transitions are invalid.
@@ -446,28 +579,303 @@ struct GroupSolver {
     }
 };
 
-template <class Grp>
+// Small automata (n <= 16) without witness: the whole live frontier sits in an
+// LDS ring of 16-bit subsets (ring_cap entries; the largest two-level window
+// of C_16 is 3099 subsets), images come from four packed nibble tables
+// (a | b << 16), and every lane takes T frontier entries per level step.  A
+// frontier that outgrows the ring moves to a 65536-entry ring in the group's
+// HBM slot, which no n <= 16 frontier can outgrow.
+template <class Grp, int T>
+struct SmallSolver {
+    static constexpr uint32_t G = Grp::kSize;
+    static constexpr int kBits = (2 * T >= 8) ? 4 : (2 * T >= 4 ? 3 : 2);
+    static constexpr uint32_t kGlobalRing = 65536;
+    static constexpr uint32_t kStage = 64;  // staging bytes for the transitions
+    const BatchParams &P;
+    Grp g;
+    lds_u32 *tab;
+    lds_u8 *stage;
+    lds_u32 *bm;
+    lds_u16 *ring;
+    glb_u16 *gring;
+    uint32_t rmask;
+    bool in_lds;
+    int64_t ai;
+    int32_t n;
+    uint32_t Q;
+    int32_t nsl;
+    uint32_t hi, wr, rd, level;
+    int64_t it, it_end;
+    uint32_t wq;
+    bool taken;
+    uint32_t done_local;
+    uint64_t subsets_local;
+    // the next automaton, loaded while the current one is searched
+    int64_t nx_ai;
+    int32_t nx_n;
+    uint32_t nx_b0, nx_b1;
+
+    __device__ SmallSolver(const BatchParams &p, void *scratch, uint8_t *gsm, uint8_t *slot)
+        : P(p), g(scratch) {
+        tab = (lds_u32 *)gsm;
+        stage = (lds_u8 *)(gsm + 64 * 4);
+        bm = (lds_u32 *)(gsm + 64 * 4 + kStage);
+        ring = (lds_u16 *)(gsm + 64 * 4 + kStage + 4u * P.lds_bitmap_words);
+        gring = (glb_u16 *)(slot + P.garena_off);
+        rmask = P.lds_arena_cap - 1u;
+        it = 0;
+        it_end = 0;
+        wq = 0;
+        taken = false;
+        done_local = 0;
+        subsets_local = 0;
+        nx_ai = -1;
+    }
+    static __host__ __device__ size_t lds_bytes(uint32_t bitmap_words, uint32_t ring_cap) {
+        return 64 * 4 + kStage + 4ull * bitmap_words + ((2ull * ring_cap + 15) & ~15ull);
+    }
+    // LDS-only ordering inside one wavefront needs no memory fence: LDS
+    // instructions of a wave complete in order
+    __device__ __forceinline__ void lsync() const {
+        if (G > 64 || !in_lds)
+            g.sync();
+        else
+            __builtin_amdgcn_wave_barrier();
+    }
+
+    template <bool LDS>
+    __device__ __forceinline__ uint32_t rget(uint32_t p) const {
+        return LDS ? (uint32_t)ring[p & rmask] : (uint32_t)gring[p & (kGlobalRing - 1u)];
+    }
+    template <bool LDS>
+    __device__ __forceinline__ void rput(uint32_t p, uint32_t v) const {
+        if (LDS)
+            ring[p & rmask] = (uint16_t)v;
+        else
+            gring[p & (kGlobalRing - 1u)] = (uint16_t)v;
+    }
+    // all four slices are built (slices past n hold zeros), so the lookups
+    // are unconditional and issue back to back
+    __device__ __forceinline__ uint32_t image(uint32_t s) const {
+        return tab[s & 15u] | tab[16u | ((s >> 4) & 15u)] | tab[32u | ((s >> 8) & 15u)] |
+               tab[48u | ((s >> 12) & 15u)];
+    }
+    __device__ __forceinline__ bool tas(uint32_t x) const {
+        const uint32_t bit = 1u << (x & 31u);
+        return (or_wg(bm + (x >> 5), bit) & bit) == 0u;
+    }
+
+    __device__ int64_t next_item() {
+        if (P.single_item) {
+            if (taken) return -1;
+            taken = true;
+            return P.first_item;
+        }
+        return fetch_work(P, g, wq, it, it_end);
+    }
+
+    __device__ void write_invalid() const {
+        P.length[ai] = -3;
+        if (P.sync) P.sync[ai] = 0;
+        if (P.expanded) P.expanded[ai] = 0;
+    }
+
+    // Takes the next automaton from the work queue and issues the loads of
+    // its size and transitions; they are consumed by start().
+    __device__ void prefetch() {
+        const int64_t item = next_item();
+        if (item < 0) {
+            nx_ai = -1;
+            return;
+        }
+        nx_ai = uni<G>(P.order ? (int64_t)P.order[item] : item);
+        nx_n = uni<G>(P.uniform_n > 0 ? P.uniform_n : (int32_t)P.n_per[nx_ai]);
+        nx_b0 = nx_b1 = 0;
+        if (nx_n >= 1 && nx_n <= P.lds_n) {
+            const uint8_t *d = P.delta + (P.offset ? P.offset[nx_ai] : (uint64_t)nx_ai * 2u * (uint64_t)nx_n);
+            const uint32_t r = g.rank();
+            if (r < 2u * (uint32_t)nx_n) nx_b0 = d[r];
+            if (G < 32 && r + G < 2u * (uint32_t)nx_n) nx_b1 = d[r + G];
+        }
+    }
+    __device__ bool has_next() const { return nx_ai >= 0; }
+
+    // Starts the prefetched automaton; false when its input is invalid.
+    __device__ bool start() {
+        ai = nx_ai;
+        n = uni<G>(nx_n);
+        const uint32_t r = g.rank();
+        const uint32_t b0 = nx_b0, b1 = nx_b1;
+        const bool bad_n = n < 1 || n > P.lds_n;
+        const uint32_t two_n = bad_n ? 0u : 2u * (uint32_t)n;
+        bool badd = (r < two_n && b0 >= (uint32_t)n) || (G < 32 && r + G < two_n && b1 >= (uint32_t)n);
+        if (r < two_n) stage[r] = (uint8_t)b0;
+        if (G < 32 && r + G < two_n) stage[r + G] = (uint8_t)b1;
+        prefetch();
+        in_lds = true;
+        if (g.any(bad_n || badd)) {
+            if (r == 0) write_invalid();
+            return false;
+        }
+        lsync();
+        nsl = (n + 3) >> 2;
+        Q = uni<G>((1u << n) - 1u);
+        const uint32_t words = n >= 5 ? (1u << (n - 5)) : 1u;
+        if (words >= 4) {
+            lds_u4 *b4 = (lds_u4 *)bm;
+            for (uint32_t w = r; w < (words >> 2); w += G) b4[w] = u32x4{0u, 0u, 0u, 0u};
+        } else {
+            for (uint32_t w = r; w < words; w += G) bm[w] = 0;
+        }
+        for (int e = (int)r; e < 64; e += (int)G) {
+            const int k = e >> 4, v = e & 15;
+            uint32_t acc = 0;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int q = 4 * k + t;
+                if (q < n && ((v >> t) & 1)) acc |= (1u << stage[q]) | (1u << (16 + stage[n + q]));
+            }
+            tab[e] = acc;
+        }
+        lsync();
+        if (r == 0) {
+            or_wg(bm + (Q >> 5), 1u << (Q & 31u));
+            ring[0] = (uint16_t)Q;
+        }
+        lsync();
+        hi = 1;
+        wr = 1;
+        rd = 0;
+        level = 0;
+        return true;
+    }
+
+    __device__ void migrate() {
+        for (uint32_t p = rd + g.rank(); p < wr; p += G) gring[p & (kGlobalRing - 1u)] = ring[p & rmask];
+        g.sync();
+        in_lds = false;
+    }
+
+    __device__ int step() {
+        if (in_lds && wr + 2u * G * T - rd > P.lds_arena_cap) migrate();
+        // one code path per ring location, so the LDS path carries no wait
+        // on outstanding HBM loads (the prefetched next automaton)
+        return in_lds ? step_in<true>() : step_in<false>();
+    }
+
+    template <bool LDS>
+    __device__ int step_in() {
+        const uint32_t r = g.rank();
+        uint32_t s[T];
+        bool v[T];
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+            const uint32_t p = rd + t * G + r;
+            v[t] = p < hi;
+            // lanes past the level read a harmless slot and are masked below
+            s[t] = rget<LDS>(v[t] ? p : rd);
+            s[t] = v[t] ? s[t] : 0u;
+        }
+        uint32_t img[T];
+#pragma unroll
+        for (int t = 0; t < T; ++t) img[t] = image(s[t]);
+        bool single = false;
+        uint32_t c = 0, newmask = 0;
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+            const uint32_t a = img[t] & 0xFFFFu, b = img[t] >> 16;
+            const bool sa = v[t] && is_single(a) && a != Q;
+            const bool sb = v[t] && is_single(b) && b != Q;
+            single |= sa | sb;
+            // both test-and-sets of a lane issue back to back; lanes past the
+            // level skip them (a singleton image tests the empty subset's bit,
+            // which no search ever sets)
+            const uint32_t xa = sa ? 0u : a, xb = sb ? 0u : b;
+            const uint32_t ba = 1u << (xa & 31u), bb = 1u << (xb & 31u);
+            uint32_t oa = ~0u, ob = ~0u;
+            if (v[t]) {
+                oa = or_wg(bm + (xa >> 5), ba);
+                ob = or_wg(bm + (xb >> 5), bb);
+            }
+            const bool na = xa != 0u && (oa & ba) == 0u;
+            const bool nb = xb != 0u && (ob & bb) == 0u && xb != xa;
+            newmask |= (na ? 1u : 0u) << (2 * t);
+            newmask |= (nb ? 1u : 0u) << (2 * t + 1);
+            c += (na ? 1u : 0u) + (nb ? 1u : 0u);
+        }
+        bool any_single;
+        uint32_t prefix, total;
+        g.template exchange_count<kBits>(single, c, any_single, prefix, total);
+        uint32_t k = wr + prefix;
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+            if ((newmask >> (2 * t)) & 1u) rput<LDS>(k++, img[t] & 0xFFFFu);
+            if ((newmask >> (2 * t + 1)) & 1u) rput<LDS>(k++, img[t] >> 16);
+        }
+        wr = uni<G>(wr + total);
+        if (uni<G>(any_single ? 1u : 0u)) return kFound;
+        rd = uni<G>(rd + G * T);
+        if (rd >= hi) {
+            if (G > 64 || !LDS)
+                g.sync();
+            else
+                __builtin_amdgcn_wave_barrier();
+            ++level;
+            rd = hi;
+            hi = wr;
+            if (rd == hi) return kNoSync;
+        }
+        return kContinue;
+    }
+
+    __device__ void finish(int code) {
+        const bool found = code == kFound;
+        const int32_t L = found ? (int32_t)level + 1 : -1;
+        const int64_t E = hi;
+        if (g.rank() == 0) {
+            P.length[ai] = L;
+            if (P.sync) P.sync[ai] = found ? 1 : 0;
+            if (P.expanded) P.expanded[ai] = E;
+            const int32_t bound = P.bound >= 0 ? P.bound : (n - 1) * (n - 1);
+            if (found && L > bound) atomicMin(P.violation, (unsigned long long)ai);
+        }
+        ++done_local;
+        subsets_local += (uint64_t)E;
+        lsync();
+        if (done_local >= 256) flush();
+    }
+
+    __device__ void flush() {
+        if (g.rank() == 0 && done_local) {
+            atomicAdd(&P.progress[0], (unsigned long long)done_local);
+            atomicAdd(&P.progress[1], (unsigned long long)subsets_local);
+        }
+        done_local = 0;
+        subsets_local = 0;
+    }
+};
+
+template <class Solver, class Grp>
 __global__ __launch_bounds__(256) void batch_kernel(BatchParams P) {
     extern __shared__ __align__(16) uint8_t smem[];
     constexpr uint32_t G = Grp::kSize;
     const uint32_t groups_per_block = blockDim.x / G;
     const uint32_t gib = threadIdx.x / G;
-    const uint32_t per_group = kTableBytes + 4u * P.lds_bitmap_words + 4u * P.lds_arena_cap;
+    const uint32_t per_group = (uint32_t)P.lds_group_bytes;
     uint8_t *gsm = smem + (uint64_t)gib * per_group;
     void *scratch = smem + (uint64_t)groups_per_block * per_group;
     const uint64_t slot = (uint64_t)blockIdx.x * groups_per_block + gib;
-    GroupSolver<Grp> S(P, scratch, gsm, P.slot_base + slot * P.slot_bytes);
+    Solver S(P, scratch, gsm, P.slot_base + slot * P.slot_bytes);
+    S.wq = (uint32_t)(slot & (kQueues - 1));
 
     bool active = true, need_new = true;
+    S.prefetch();
     for (;;) {
         if (active && need_new) {
-            const int64_t item = S.next_item();
-            if (item < 0) {
+            if (!S.has_next())
                 active = false;
-            } else {
-                const int64_t a = P.order ? (int64_t)P.order[item] : item;
-                need_new = !S.init(a);
-            }
+            else
+                need_new = !S.start();
         }
         if (G < 64) {
             if (__ballot(active) == 0) break;
@@ -499,41 +907,60 @@ __global__ void fill_u32_kernel(uint32_t *p, uint64_t words, uint32_t value) {
 
 namespace ssw {
 
-template <class Grp>
-static hipError_t launch_batch_t(const BatchParams &P, int grid, int block, size_t lds,
-                                 hipStream_t st) {
-    hipLaunchKernelGGL(batch_kernel<Grp>, dim3(grid), dim3(block), lds, st, P);
+template <class Solver, class Grp>
+static hipError_t launch_t(const BatchParams &P, int grid, int block, size_t lds, hipStream_t st) {
+    hipLaunchKernelGGL((batch_kernel<Solver, Grp>), dim3(grid), dim3(block), lds, st, P);
     return hipGetLastError();
 }
 
-template <class Grp>
-static hipError_t occupancy_t(int block, size_t lds, int *blocks) {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, batch_kernel<Grp>, block, lds);
+template <class Solver, class Grp>
+static hipError_t occ_t(int block, size_t lds, int *blocks) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, batch_kernel<Solver, Grp>, block, lds);
 }
 
-hipError_t launch_batch(int group_lanes, const BatchParams &P, int grid, int block, size_t lds,
-                        hipStream_t st) {
-    switch (group_lanes) {
-        case 16: return launch_batch_t<WaveGroup<16>>(P, grid, block, lds, st);
-        case 32: return launch_batch_t<WaveGroup<32>>(P, grid, block, lds, st);
-        case 64: return launch_batch_t<WaveGroup<64>>(P, grid, block, lds, st);
-        case 256: return launch_batch_t<BlockGroup<256>>(P, grid, block, lds, st);
-        default: return hipErrorInvalidValue;
+// variant: 0 = general (witness, HBM bitmaps), 1..3 = small path with T = 1, 2, 4
+#define SSW_DISPATCH(FN, ...)                                                              \
+    switch (variant * 1000 + group_lanes) {                                                \
+        case 16: return FN<GroupSolver<WaveGroup<16>>, WaveGroup<16>>(__VA_ARGS__);         \
+        case 32: return FN<GroupSolver<WaveGroup<32>>, WaveGroup<32>>(__VA_ARGS__);         \
+        case 64: return FN<GroupSolver<WaveGroup<64>>, WaveGroup<64>>(__VA_ARGS__);         \
+        case 256: return FN<GroupSolver<BlockGroup<256>>, BlockGroup<256>>(__VA_ARGS__);    \
+        case 1016: return FN<SmallSolver<WaveGroup<16>, 1>, WaveGroup<16>>(__VA_ARGS__);    \
+        case 1032: return FN<SmallSolver<WaveGroup<32>, 1>, WaveGroup<32>>(__VA_ARGS__);    \
+        case 1064: return FN<SmallSolver<WaveGroup<64>, 1>, WaveGroup<64>>(__VA_ARGS__);    \
+        case 1256: return FN<SmallSolver<BlockGroup<256>, 1>, BlockGroup<256>>(__VA_ARGS__); \
+        case 2016: return FN<SmallSolver<WaveGroup<16>, 2>, WaveGroup<16>>(__VA_ARGS__);    \
+        case 2032: return FN<SmallSolver<WaveGroup<32>, 2>, WaveGroup<32>>(__VA_ARGS__);    \
+        case 2064: return FN<SmallSolver<WaveGroup<64>, 2>, WaveGroup<64>>(__VA_ARGS__);    \
+        case 2256: return FN<SmallSolver<BlockGroup<256>, 2>, BlockGroup<256>>(__VA_ARGS__); \
+        case 3016: return FN<SmallSolver<WaveGroup<16>, 4>, WaveGroup<16>>(__VA_ARGS__);    \
+        case 3032: return FN<SmallSolver<WaveGroup<32>, 4>, WaveGroup<32>>(__VA_ARGS__);    \
+        case 3064: return FN<SmallSolver<WaveGroup<64>, 4>, WaveGroup<64>>(__VA_ARGS__);    \
+        case 3256: return FN<SmallSolver<BlockGroup<256>, 4>, BlockGroup<256>>(__VA_ARGS__); \
+        default: return hipErrorInvalidValue;                                              \
     }
+
+hipError_t launch_batch(int variant, int group_lanes, const BatchParams &P, int grid, int block,
+                        size_t lds, hipStream_t st) {
+    SSW_DISPATCH(launch_t, P, grid, block, lds, st)
 }
 
-hipError_t batch_occupancy(int group_lanes, int block, size_t lds, int *blocks) {
-    switch (group_lanes) {
-        case 16: return occupancy_t<WaveGroup<16>>(block, lds, blocks);
-        case 32: return occupancy_t<WaveGroup<32>>(block, lds, blocks);
-        case 64: return occupancy_t<WaveGroup<64>>(block, lds, blocks);
-        case 256: return occupancy_t<BlockGroup<256>>(block, lds, blocks);
-        default: return hipErrorInvalidValue;
-    }
+hipError_t batch_occupancy(int variant, int group_lanes, int block, size_t lds, int *blocks) {
+    SSW_DISPATCH(occ_t, block, lds, blocks)
 }
 
 size_t batch_scratch_bytes(int group_lanes) {
     return group_lanes == 256 ? (size_t)BlockGroup<256>::kScratchBytes : 0;
+}
+
+size_t batch_group_lds(int variant, uint32_t bitmap_words, uint32_t arena_cap) {
+    return variant == 0 ? (size_t)(kTableBytes + 4ull * bitmap_words + 4ull * arena_cap)
+                        : SmallSolver<WaveGroup<64>, 1>::lds_bytes(bitmap_words, arena_cap);
+}
+
+hipError_t init_queues(uint32_t *queues, int64_t count, hipStream_t st) {
+    hipLaunchKernelGGL(init_queues_kernel, dim3(1), dim3(kQueues), 0, st, queues, count);
+    return hipGetLastError();
 }
 
 hipError_t fill_u32(uint32_t *p, uint64_t words, uint32_t value, hipStream_t st) {

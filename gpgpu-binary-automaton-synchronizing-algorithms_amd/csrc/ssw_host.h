@@ -70,15 +70,53 @@ struct ssw_context;
 
 namespace ssw {
 
+// Host mirror of one device-wide instance slot.
+struct LargeSlot {
+    bool used = false;
+    bool harvested = false;
+    bool append = false;
+    int64_t index = -1;
+    int n = 0;
+    uint32_t *ring = nullptr;
+    uint64_t ring_cap = 0;
+    uint32_t *bitmap = nullptr;
+    uint64_t *table = nullptr;
+    uint64_t *lvl = nullptr;
+    uint32_t lvl_cap = 0;
+    uint64_t bytes = 0;
+};
+
 // Device-wide level loop for large instances (ssw_large.cpp).
 struct LargeEngine {
-    DevBuf state;        // LargeState + LargeInstance[]
-    DevBuf pool;         // bitmaps, rings, tables of the instances
+    DevBuf state;                        // LargeState + LargeInstance[kLargeMaxInstances]
+    std::vector<LargeSlot> slots;
+    std::vector<LargeInstance> hinst;    // host copy of the instance array
+    LargeState hstate;                   // staging for the output pointers
+    int num = 0;                         // slots in use on the device: 0..num-1
+    int grid = 1024;
+    int tick_levels = 32;
+    uint64_t bytes_in_use = 0;
     hipGraphExec_t graph = nullptr;
     int graph_levels = 0;
-    int graph_grid = 0;
-    void *graph_state = nullptr;
+
+    LargeState *dstate() const;
+    LargeInstance *dinst() const;
     void release();
+    void free_slot_buffers(LargeSlot &s);
+    int setup(ssw_context &ctx);
+    int set_outputs(ssw_context &ctx, int32_t *length, uint8_t *sync, int64_t *expanded,
+                    unsigned long long *violation, unsigned long long *progress);
+    static uint64_t admit_bytes(int n, bool append, uint64_t ring_cap);
+    // Admits one automaton; *slot_out = -1 when no slot or memory is free now.
+    int admit(ssw_context &ctx, int64_t index, int n, const uint8_t *delta, const uint64_t *offset,
+              int32_t bound, bool report, bool append, uint64_t budget, int *slot_out);
+    int build_graph(ssw_context &ctx, int levels);
+    int tick(ssw_context &ctx, int levels);
+    int read_back(ssw_context &ctx);
+    int running() const;
+    int witness(ssw_context &ctx, int slot, uint8_t *out, int stride);
+    void retire(int slot);
+    int clear_slot(ssw_context &ctx, int slot);
     // Solves the listed automata of db (indices into db) concurrently.
     int solve_indices(ssw_context &ctx, const DeviceBatch &db, const uint8_t *host_n,
                       const std::vector<uint32_t> &idx);
@@ -91,6 +129,9 @@ struct ssw_context {
     int device = -1;
     int cus = 0;
     size_t lds_per_cu = 0;
+    size_t lds_per_block = 65536;
+    int last_grid = 0, last_block = 0, last_variant = 0;
+    size_t last_lds = 0;
     hipStream_t stream = nullptr;
     hipStream_t monitor = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -99,7 +140,7 @@ struct ssw_context {
     std::mutex monitor_mu;   // progress reads
     std::string err;
 
-    ssw::DevBuf ctrs, deferred, order;
+    ssw::DevBuf ctrs, queues, deferred, order;
     ssw::DevBuf d_delta, d_nper, d_off, d_len, d_sync, d_exp, d_wit;
     std::map<ssw::SlotKey, ssw::DevBuf> slot_cache;
     ssw::LargeEngine large;

@@ -13,6 +13,12 @@ constexpr int kSliceBits = 4;
 constexpr int kTableEntries = 8 * 16;
 constexpr int kTableBytes = kTableEntries * 8;
 
+// Device work queue: kQueues segments of kQueueWords u32 each (next, end),
+// then a u64 mask of exhausted segments.
+constexpr int kQueues = 64;
+constexpr int kQueueWords = 32;
+constexpr size_t kQueueBytes = kQueues * kQueueWords * 4 + 64;
+
 // Result codes of one BFS level step.
 enum StepCode : int { kContinue = 0, kFound = 1, kNoSync = 2, kOverflow = 3 };
 
@@ -33,7 +39,7 @@ struct BatchParams {
     int32_t witness_stride;
     uint32_t want_witness;
     unsigned long long *violation;     // atomicMin of violating indices
-    unsigned int *work_counter;        // device work queue head
+    uint32_t *queues;                  // kQueues segment counters + exhausted mask
     unsigned int *deferred_count;
     uint32_t *deferred_list;           // automata whose frontier outgrew the arena
     unsigned long long *progress;      // [0] automata done, [1] subsets expanded
@@ -49,38 +55,65 @@ struct BatchParams {
     uint32_t lvl_cap;
     int32_t lds_n;                     // the LDS bitmap serves n <= lds_n
     uint32_t lds_bitmap_words;         // per group, multiple of 4
-    uint32_t lds_arena_cap;            // per group, multiple of 4
+    uint32_t lds_arena_cap;            // per group: u32 arena entries, or u16 ring entries (power of 2)
+    uint32_t lds_group_bytes;          // LDS bytes of one group
     uint32_t chunk;                    // automata per work-queue fetch
     uint32_t single_item;              // plain path: solve only first_item
     int64_t first_item;
 };
 
-// Device-wide path: one instance per automaton, all in HBM.
+// Device-wide path: every instance keeps its visited bitmap (2^n bits) and a
+// frontier ring in HBM; one launch expands the current level of all running
+// instances.
+constexpr int kLargeTile = 1024;          // frontier entries per workgroup tile
+constexpr int kLargeMaxInstances = 256;   // concurrent instances per engine
+
+enum LargeStatus : int32_t {
+    kLIdle = 0,      // free slot
+    kLRun = 1,
+    kLPause = 2,     // ring (or level table) too small for the next level: host grows it
+    kLFound = 3,
+    kLNoSync = 4,
+    kLError = 5,     // invalid transitions or ring overflow
+};
+
 struct LargeInstance {
-    int32_t n;
-    int32_t status;          // 0 running, 1 found, 2 not synchronizing, 3 overflow
-    int32_t level;           // level being expanded
-    int32_t length;
-    uint64_t lo, hi;         // current level [lo, hi) in ring positions
-    uint64_t wr;             // write position (atomic during a step)
-    uint64_t ring_mask;      // ring capacity - 1 (power of two)
-    uint32_t *ring;          // frontier ring
+    // set by the host
+    uint32_t *ring;          // frontier ring of u32 subsets, capacity ring_mask + 1
     uint32_t *bitmap;        // 2^n bits
-    uint64_t *table;         // 4 byte slices x 256 entries (a | b << 32)
+    uint64_t *table;         // 4 byte slices x 256 entries: a | b << 32
+    uint64_t *lvl;           // append mode: start position of every level
+    uint64_t ring_mask;
     int64_t index;           // automaton index in the caller's batch
-    int32_t found_flag;      // set by any thread that meets a singleton
+    int32_t n;
     int32_t bound;
-    uint64_t tile_base;      // first tile of this instance in the flattened grid
+    uint32_t lvl_cap;
+    uint32_t append;         // 1: positions never wrap (the witness needs every level)
+    // device state
+    uint64_t lo, hi;         // level being expanded: positions [lo, hi)
+    unsigned long long wr;   // next free position
+    int32_t level;
+    int32_t status;
+    int32_t found;           // a singleton image was met in this level
+    int32_t overflow;
+    int32_t length;
+    int32_t report;          // 1: write the result into the batch outputs
     uint64_t expanded;
+    uint64_t pad[2];
 };
 
 struct LargeState {
-    int32_t num_instances;
-    int32_t active;          // instances still running
-    uint32_t blocks_done;    // last-block-done counter of a step
-    uint32_t step_count;
+    int32_t num;             // slots in use (0..num-1)
+    int32_t active;          // running instances with work this level
     uint64_t total_tiles;
-    unsigned long long steps_executed;
+    int32_t act[kLargeMaxInstances];
+    uint64_t act_base[kLargeMaxInstances + 1];
+    // batch outputs (may be null)
+    int32_t *length;
+    uint8_t *sync;
+    int64_t *expanded;
+    unsigned long long *violation;
+    unsigned long long *progress;
 };
 
 }  // namespace ssw

@@ -22,11 +22,13 @@
 #include <vector>
 
 namespace ssw {
-hipError_t launch_batch(int group_lanes, const BatchParams &P, int grid, int block, size_t lds,
-                        hipStream_t st);
-hipError_t batch_occupancy(int group_lanes, int block, size_t lds, int *blocks);
+hipError_t launch_batch(int variant, int group_lanes, const BatchParams &P, int grid, int block,
+                        size_t lds, hipStream_t st);
+hipError_t batch_occupancy(int variant, int group_lanes, int block, size_t lds, int *blocks);
 size_t batch_scratch_bytes(int group_lanes);
+size_t batch_group_lds(int variant, uint32_t bitmap_words, uint32_t arena_cap);
 hipError_t fill_u32(uint32_t *p, uint64_t words, uint32_t value, hipStream_t st);
+hipError_t init_queues(uint32_t *queues, int64_t count, hipStream_t st);
 }  // namespace ssw
 
 using namespace ssw;
@@ -87,6 +89,7 @@ int ssw_context::open(int dev, uint64_t mem_budget) {
     HIP_TRY(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
     cus = prop.multiProcessorCount;
     lds_per_cu = prop.maxSharedMemoryPerMultiProcessor;
+    lds_per_block = prop.sharedMemPerBlock;
     HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
     HIP_TRY(hipStreamCreateWithFlags(&monitor, hipStreamNonBlocking), "hipStreamCreate");
     HIP_TRY(hipEventCreate(&ev0), "hipEventCreate");
@@ -95,6 +98,8 @@ int ssw_context::open(int dev, uint64_t mem_budget) {
     HIP_TRY(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
     budget = mem_budget ? mem_budget : (uint64_t)(0.8 * (double)free_b);
     int rc = ensure(ctrs, kCtrBytes, "counters");
+    if (rc) return rc;
+    rc = ensure(queues, kQueueBytes, "work queues");
     if (rc) return rc;
     HIP_TRY(hipMemsetAsync(ctrs.p, 0, kCtrBytes, stream), "hipMemset");
     HIP_TRY(hipStreamSynchronize(stream), "hipStreamSynchronize");
@@ -105,7 +110,7 @@ int ssw_context::open(int dev, uint64_t mem_budget) {
 void ssw_context::close() {
     stop_progress_thread();
     if (device >= 0) hipSetDevice(device);
-    for (DevBuf *b : {&ctrs, &deferred, &order, &d_delta, &d_nper, &d_off, &d_len, &d_sync, &d_exp,
+    for (DevBuf *b : {&ctrs, &queues, &deferred, &order, &d_delta, &d_nper, &d_off, &d_len, &d_sync, &d_exp,
                       &d_wit})
         if (b->p) hipFree(b->p);
     for (auto &kv : slot_cache)
@@ -122,13 +127,13 @@ void ssw_context::default_tuning() {
     for (int n = 0; n <= SSW_MAX_N; ++n) {
         ssw_tuning t;
         if (n <= 10) {
-            t.group_lanes = 16; t.block_threads = 256; t.chunk = 16; t.arena_lds = 256;
+            t.group_lanes = 16; t.block_threads = 256; t.chunk = 16; t.arena_lds = 1024; t.lane_entries = 1;
         } else if (n <= 13) {
-            t.group_lanes = 32; t.block_threads = 256; t.chunk = 8; t.arena_lds = 512;
+            t.group_lanes = 32; t.block_threads = 256; t.chunk = 8; t.arena_lds = 2048; t.lane_entries = 1;
         } else if (n <= 16) {
-            t.group_lanes = 64; t.block_threads = 256; t.chunk = 4; t.arena_lds = 1024;
+            t.group_lanes = 64; t.block_threads = 0; t.chunk = 2; t.arena_lds = 2048; t.lane_entries = 1;
         } else {
-            t.group_lanes = 64; t.block_threads = 256; t.chunk = 2; t.arena_lds = 2048;
+            t.group_lanes = 64; t.block_threads = 256; t.chunk = 2; t.arena_lds = 2048; t.lane_entries = 1;
         }
         tune[n] = t;
     }
@@ -170,14 +175,23 @@ int ssw_context::get_slots(const SlotKey &key, uint64_t bytes, DevBuf **out) {
 
 static inline uint64_t align_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
 
+static inline uint32_t next_pow2(uint64_t x) {
+    uint64_t p = 1;
+    while (p < x) p <<= 1;
+    return (uint32_t)p;
+}
+
 int ssw_context::launch_class(const DeviceBatch &db, const ClassSpec &cls, bool plain) {
-    const ssw_tuning t = plain ? ssw_tuning{256, 256, 1, 1024} : tune[cls.nmax];
+    const ssw_tuning t = plain ? ssw_tuning{256, 256, 1, 8192, 1} : tune[cls.nmax];
     const int G = t.group_lanes;
-    const int block = G == 256 ? 256 : std::max(64, std::min(256, t.block_threads));
-    const int groups_per_block = block / G;
+    int block = G == 256 ? 256 : std::max(64, std::min(256, t.block_threads ? t.block_threads : 256));
     const bool use_lds = cls.lds;
     const int nmax = cls.nmax;
     const bool want_wit = db.witness != nullptr;
+    // the small path: bitmap in LDS, no witness, 16-bit ring frontier
+    const bool small = use_lds && !want_wit && nmax <= kMaxLdsN;
+    const int T = t.lane_entries >= 4 ? 4 : (t.lane_entries >= 2 ? 2 : 1);
+    const int variant = small ? (T == 1 ? 1 : (T == 2 ? 2 : 3)) : 0;
 
     BatchParams P;
     std::memset(&P, 0, sizeof P);
@@ -195,7 +209,7 @@ int ssw_context::launch_class(const DeviceBatch &db, const ClassSpec &cls, bool 
     P.witness_stride = db.witness_stride;
     P.want_witness = want_wit ? 1u : 0u;
     uint8_t *c = (uint8_t *)ctrs.p;
-    P.work_counter = (unsigned int *)(c + kCtrWork);
+    P.queues = (uint32_t *)queues.p;
     P.deferred_count = (unsigned int *)(c + kCtrDeferred);
     P.violation = (unsigned long long *)(c + kCtrViolation);
     P.progress = (unsigned long long *)(c + kCtrProgress);
@@ -204,45 +218,82 @@ int ssw_context::launch_class(const DeviceBatch &db, const ClassSpec &cls, bool 
     P.lds_n = use_lds ? nmax : 0;
     const uint32_t bm_words = use_lds ? (uint32_t)align_up(nmax >= 5 ? (1u << (nmax - 5)) : 1u, 4) : 4u;
     P.lds_bitmap_words = bm_words;
-    uint32_t arena_lds = (uint32_t)align_up((uint64_t)std::max(4, t.arena_lds), 4);
-    // the LDS arena never needs more than every subset of Q
     const uint64_t all_subsets = 1ull << nmax;
-    if (arena_lds > all_subsets + 2u * G) arena_lds = (uint32_t)align_up(all_subsets + 2u * G, 4);
+    uint32_t arena_lds;
+    if (small) {
+        // a ring holding every subset of Q never has to leave LDS
+        arena_lds = next_pow2((uint64_t)std::max(t.arena_lds, 4 * G * T));
+        arena_lds = std::min(arena_lds, next_pow2(all_subsets + 2ull * G * T));
+    } else {
+        arena_lds = (uint32_t)align_up((uint64_t)std::max(4, t.arena_lds), 4);
+        if (arena_lds > all_subsets + 2u * G) arena_lds = (uint32_t)align_up(all_subsets + 2u * G, 4);
+    }
     P.lds_arena_cap = arena_lds;
-    const size_t per_group = kTableBytes + 4ull * bm_words + 4ull * arena_lds;
+    const size_t per_group = batch_group_lds(variant, bm_words, arena_lds);
+    P.lds_group_bytes = (uint32_t)per_group;
+    // workgroup size: the tuned one, or (block_threads = 0) the one that keeps
+    // the most groups resident per CU under the LDS budget
+    int per_cu = 0;
+    if (!plain && G < 256 && t.block_threads == 0) {
+        int best_groups = 0;
+        for (int cand = 64; cand <= 256; cand += 64) {
+            if (cand % G) continue;
+            const size_t l = per_group * (cand / G) + batch_scratch_bytes(G);
+            if (l > lds_per_block) continue;
+            int occ = 0;
+            HIP_TRY(batch_occupancy(variant, G, cand, l, &occ), "occupancy");
+            if (occ * (cand / G) >= best_groups && occ > 0) {
+                best_groups = occ * (cand / G);
+                block = cand;
+                per_cu = occ;
+            }
+        }
+    } else {
+        // fewer groups per workgroup when the tuned block does not fit the LDS limit
+        while (G < 256 && block > 64 && per_group * (block / G) > lds_per_block) block -= 64;
+    }
+    const int groups_per_block = block / G;
     const size_t lds = per_group * groups_per_block + batch_scratch_bytes(G);
-    if (lds > 65536) return fail(SSW_ERR_ARG, "tuning for n=%d needs %zu bytes of LDS", nmax, lds);
-
-    // global arena: every subset fits when n <= 24 or the bitmap is in LDS;
-    // larger automata whose frontier outgrows it are deferred
-    uint64_t garena_cap;
-    if (use_lds || nmax <= 20)
-        garena_cap = all_subsets + 2u * G > arena_lds ? all_subsets + 2u * G - arena_lds : 0;
-    else
-        garena_cap = plain ? (1ull << 22) : (1ull << 18);
-    const uint64_t gbm_words = use_lds ? 0 : std::max<uint64_t>(1, all_subsets / 32);
-    const uint64_t good_words = want_wit ? std::max<uint64_t>(1, all_subsets / 32) : 0;
-    const uint32_t lvl_cap = want_wit ? (uint32_t)((nmax - 1) * (nmax - 1) + 8) : 0;
+    if (lds > lds_per_block)
+        return fail(SSW_ERR_ARG, "tuning for n=%d needs %zu bytes of LDS per block (limit %zu)", nmax, lds,
+                    lds_per_block);
 
     uint64_t off = 0;
-    P.garena_off = off;
-    P.garena_cap = garena_cap;
-    off = align_up(off + 4 * garena_cap, 256);
-    P.gbitmap_off = off;
-    P.gbitmap_words = gbm_words;
-    off = align_up(off + 4 * gbm_words, 256);
-    P.good_off = off;
-    P.good_words = good_words;
-    off = align_up(off + 8 * good_words, 256);
-    P.lvl_off = off;
-    P.lvl_cap = lvl_cap;
-    off = align_up(off + 4ull * lvl_cap, 256);
+    if (small) {
+        // 16-bit HBM ring for frontiers that outgrow the LDS ring
+        P.garena_off = 0;
+        P.garena_cap = 65536;
+        off = align_up(2ull * 65536, 256);
+        P.gbitmap_off = P.good_off = P.lvl_off = off;
+    } else {
+        // every subset fits the HBM arena when n <= 20; larger automata whose
+        // frontier outgrows it are deferred to the device-wide path
+        uint64_t garena_cap;
+        if (use_lds || nmax <= 20)
+            garena_cap = all_subsets + 2u * G > arena_lds ? all_subsets + 2u * G - arena_lds : 0;
+        else
+            garena_cap = plain ? (1ull << 22) : (1ull << 18);
+        const uint64_t gbm_words = use_lds ? 0 : std::max<uint64_t>(1, all_subsets / 32);
+        const uint64_t good_words = want_wit ? std::max<uint64_t>(1, all_subsets / 32) : 0;
+        const uint32_t lvl_cap = want_wit ? (uint32_t)((nmax - 1) * (nmax - 1) + 8) : 0;
+        P.garena_off = off;
+        P.garena_cap = garena_cap;
+        off = align_up(off + 4 * garena_cap, 256);
+        P.gbitmap_off = off;
+        P.gbitmap_words = gbm_words;
+        off = align_up(off + 4 * gbm_words, 256);
+        P.good_off = off;
+        P.good_words = good_words;
+        off = align_up(off + 8 * good_words, 256);
+        P.lvl_off = off;
+        P.lvl_cap = lvl_cap;
+        off = align_up(off + 4ull * lvl_cap, 256);
+    }
     P.slot_bytes = std::max<uint64_t>(off, 256);
 
     int grid = 1;
     if (!plain) {
-        int per_cu = 0;
-        HIP_TRY(batch_occupancy(G, block, lds, &per_cu), "occupancy");
+        if (per_cu == 0) HIP_TRY(batch_occupancy(variant, G, block, lds, &per_cu), "occupancy");
         if (per_cu < 1) return fail(SSW_ERR_ARG, "kernel does not fit a CU (lds %zu)", lds);
         grid = per_cu * cus;
         const int64_t need = (P.count + groups_per_block - 1) / groups_per_block;
@@ -252,25 +303,30 @@ int ssw_context::launch_class(const DeviceBatch &db, const ClassSpec &cls, bool 
         while ((uint64_t)grid * groups_per_block > max_slots && grid > 1) grid = grid / 2;
     }
     const uint64_t slots = (uint64_t)grid * groups_per_block;
-    SlotKey key{G, block, P.lds_n, nmax, (int)want_wit, (int)arena_lds, plain ? 1 : 0};
+    SlotKey key{G, block, P.lds_n, nmax, (int)want_wit, (int)arena_lds, (plain ? 1 : 0) + 2 * variant};
     DevBuf *sb = nullptr;
     int rc = get_slots(key, slots * P.slot_bytes, &sb);
     if (rc) return rc;
     P.slot_base = (uint8_t *)sb->p;
+    last_grid = grid;
+    last_block = block;
+    last_lds = lds;
+    last_variant = variant;
 
     if (!plain) {
-        HIP_TRY(fill_u32((uint32_t *)P.work_counter, 1, 0, stream), "fill");
-        HIP_TRY(launch_batch(G, P, grid, block, lds, stream), "batch kernel launch");
+        HIP_TRY(init_queues(P.queues, P.count, stream), "init queues");
+        HIP_TRY(launch_batch(variant, G, P, grid, block, lds, stream), "batch kernel launch");
         ++launches;
     } else {
         for (int64_t i = 0; i < db.count; ++i) {
             if (cls.host_n && cls.host_n[i] != cls.plain_n) continue;
             P.single_item = 1;
             P.first_item = i;
-            HIP_TRY(launch_batch(G, P, 1, block, lds, stream), "plain kernel launch");
+            HIP_TRY(launch_batch(variant, G, P, 1, block, lds, stream), "plain kernel launch");
             ++launches;
         }
     }
+    total_launches += launches;
     return SSW_OK;
 }
 
@@ -672,10 +728,13 @@ int ssw_set_tuning(ssw_context *ctx, int32_t n, const ssw_tuning *t) {
     if (n < 1 || n > SSW_MAX_N || !t) return ctx->fail(SSW_ERR_ARG, "bad n or null");
     const int G = t->group_lanes;
     if (G != 16 && G != 32 && G != 64 && G != 256) return ctx->fail(SSW_ERR_ARG, "group_lanes %d", G);
-    if (t->block_threads % 64 || t->block_threads < 64 || t->block_threads > 256 ||
-        (G != 256 && t->block_threads % G) || (G == 256 && t->block_threads != 256))
+    if (t->block_threads != 0 &&
+        (t->block_threads % 64 || t->block_threads < 64 || t->block_threads > 256 ||
+         (G != 256 && t->block_threads % G) || (G == 256 && t->block_threads != 256)))
         return ctx->fail(SSW_ERR_ARG, "block_threads %d", t->block_threads);
     if (t->chunk < 1 || t->arena_lds < 4) return ctx->fail(SSW_ERR_ARG, "chunk/arena_lds");
+    if (t->lane_entries != 1 && t->lane_entries != 2 && t->lane_entries != 4)
+        return ctx->fail(SSW_ERR_ARG, "lane_entries %d", t->lane_entries);
     ctx->tune[n] = *t;
     return SSW_OK;
 }

@@ -58,7 +58,7 @@ class Batch(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [("group_lanes", ctypes.c_int32), ("block_threads", ctypes.c_int32),
-                ("chunk", ctypes.c_int32), ("arena_lds", ctypes.c_int32)]
+                ("chunk", ctypes.c_int32), ("arena_lds", ctypes.c_int32), ("lane_entries", ctypes.c_int32)]
 
 
 class Progress(ctypes.Structure):
@@ -199,6 +199,8 @@ def witness_bits_to_word(row, L):
     """Unpacks a bit-packed witness row into a uint8 array of letters (1 = b)."""
     if L <= 0:
         return np.zeros(0, np.uint8)
+    if isinstance(row, (bytes, bytearray)):
+        row = np.frombuffer(row, np.uint8)
     bits = np.unpackbits(np.asarray(row, np.uint8), bitorder="little")
     return bits[:L].astype(np.uint8)
 
@@ -334,6 +336,9 @@ class Context:
         self._check(fn(self.h, ctypes.byref(batch), SSW_DEVICE, ctypes.byref(viol)), f"solve_{mode}")
         return viol.value
 
+    def synchronize(self):
+        self._check(self._L.ssw_synchronize(self.h), "synchronize")
+
     def last_kernel_time(self):
         ms = ctypes.c_double()
         n = ctypes.c_int64()
@@ -345,10 +350,10 @@ class Context:
         t = Tuning()
         self._check(self._L.ssw_get_tuning(self.h, n, ctypes.byref(t)), "get_tuning")
         return dict(group_lanes=t.group_lanes, block_threads=t.block_threads, chunk=t.chunk,
-                    arena_lds=t.arena_lds)
+                    arena_lds=t.arena_lds, lane_entries=t.lane_entries)
 
-    def set_tuning(self, n, group_lanes, block_threads, chunk, arena_lds):
-        t = Tuning(group_lanes, block_threads, chunk, arena_lds)
+    def set_tuning(self, n, group_lanes, block_threads, chunk, arena_lds, lane_entries=1):
+        t = Tuning(group_lanes, block_threads, chunk, arena_lds, lane_entries)
         self._check(self._L.ssw_set_tuning(self.h, n, ctypes.byref(t)), "set_tuning")
 
     def autotune(self, n_lo, n_hi, samples=20000, seed=7, out_path=None):

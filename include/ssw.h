@@ -142,9 +142,10 @@ int ssw_solve_large(ssw_context *ctx, const ssw_batch *b, int32_t memory,
 
 typedef struct ssw_tuning {
     int32_t group_lanes;      /* threads per automaton: 16, 32, 64 or 256 */
-    int32_t block_threads;    /* workgroup size (multiple of 64, <= 256) */
+    int32_t block_threads;    /* workgroup size (multiple of 64, <= 256); 0 = most groups per CU */
     int32_t chunk;            /* automata taken per work-queue fetch */
     int32_t arena_lds;        /* frontier entries kept in LDS per automaton */
+    int32_t lane_entries;     /* frontier entries per lane per level step: 1, 2 or 4 */
 } ssw_tuning;
 
 int ssw_get_tuning(ssw_context *ctx, int32_t n, ssw_tuning *out);

@@ -1,0 +1,79 @@
+"""Serving loop (continuous batching): results in a mixed stream equal the
+results of the same requests solved alone, and the oracle's."""
+import time
+
+import numpy as np
+import pytest
+
+import oracle as O
+import ssw
+import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _stream(count, n_hi, seed=7):
+    reqs = W.serving(seed=seed, rate=2000.0, count=count, n_lo=6, n_hi=n_hi, cerny_frac=0.2)
+    return reqs
+
+
+def test_mixed_stream_equals_alone(ctx):
+    reqs = _stream(160, 24)
+    with ssw.Server(ctx, small_n_max=16, levels_per_tick=8, want_witness=True) as srv:
+        t0 = time.perf_counter()
+        tickets = []
+        for (t, n, a, b, kind) in reqs:
+            # replay the Poisson arrival times so requests join while others run
+            while time.perf_counter() - t0 < t:
+                time.sleep(0.0002)
+            tickets.append(srv.submit(n, a, b))
+        mixed = [srv.wait(tk, 120000) for tk in tickets]
+        st = srv.stats()
+    assert st["completed"] == len(reqs)
+    assert st["p99_us"] >= st["p50_us"] > 0
+    # alone: a fresh server, one request at a time
+    with ssw.Server(ctx, small_n_max=16, levels_per_tick=8, want_witness=True) as srv:
+        for k in range(0, len(reqs), 7):
+            _, n, a, b, _ = reqs[k]
+            alone = srv.wait(srv.submit(n, a, b), 120000)
+            m = mixed[k]
+            assert (alone["length"], alone["expanded"]) == (m["length"], m["expanded"]), f"request {k}"
+            if m["length"] >= 0:
+                np.testing.assert_array_equal(alone["witness"], m["witness"])
+    # and the oracle
+    for k, (_, n, a, b, kind) in enumerate(reqs):
+        L, S, E, _ = O.solve(n, np.asarray(a)[None], np.asarray(b)[None])
+        assert (mixed[k]["length"], mixed[k]["expanded"]) == (L[0], E[0]), f"request {k} n={n} {kind}"
+        if L[0] >= 0:
+            assert O.verify_word(n, a, b, mixed[k]["witness"])
+            if n <= 20:
+                np.testing.assert_array_equal(mixed[k]["witness"], O.witness(n, a, b)[1])
+
+
+def test_admission_rejects_oversized(ctx):
+    with ssw.Server(ctx, mem_budget=1 << 20, small_n_max=16) as srv:
+        a, b = ssw.generate_cerny(30, 1)
+        with pytest.raises(ssw.SswError) as e:
+            srv.submit(30, a, b)
+        assert e.value.code == -4
+        a, b = ssw.generate_cerny(12, 1)
+        r = srv.wait(srv.submit(12, a, b))
+        assert r["length"] == 121
+        assert srv.stats()["rejected"] == 1
+
+
+def test_large_requests_overlap(ctx):
+    # many large requests in flight at once all finish, each exact
+    reqs = []
+    for i in range(12):
+        n = 18 + (i % 5)
+        A, B = ssw.generate_uniform(300 + i, n, 1)
+        reqs.append((n, A[0], B[0]))
+    a, b = ssw.generate_cerny(21, 1)
+    reqs.append((21, a, b))
+    with ssw.Server(ctx, small_n_max=12, levels_per_tick=4, max_large=16) as srv:
+        tickets = [srv.submit(n, a, b) for (n, a, b) in reqs]
+        res = [srv.wait(t, 120000) for t in tickets]
+    for (n, a, b), r in zip(reqs, res):
+        L, S, E, _ = O.solve(n, np.asarray(a)[None], np.asarray(b)[None])
+        assert (r["length"], r["expanded"]) == (L[0], E[0])
