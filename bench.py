@@ -1,14 +1,24 @@
 """Benchmark: batched shortest-synchronizing-word search on one MI355X.
 
-Workload (one "step" = one pass of the solver over one batch, inputs already
-resident in HBM): 1,000,000 uniformly random binary automata with n = 16,
-1 % replaced by Cerny C_16 (length 225) at seeded positions.  Reports
-automata/s (value), subsets expanded/s, the plain one-automaton-per-launch GPU
-BFS on a sample of the same workload, the HBM roofline of the batched kernel,
-and the CPU oracle (a C restatement of the reference's SlimCPUUntweaked)
-timed on a bounded sample on the host cores.
+Headline workload (one "step" = one pass of the solver over one batch, inputs
+already resident in HBM): 1,000,000 uniformly random binary automata with
+n = 16, 1 % replaced by Cerny C_16 (length 225) at seeded positions; lengths,
+flags and expanded-subset counts.  `value` is automata/s of that step.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W]
+Also reported on the same JSON line:
+  * subsets expanded/s of the step, and the plain one-automaton-per-launch GPU
+    BFS on a sample of the same batch (batched_vs_plain);
+  * the HBM roofline of the batched kernel: algorithmic bytes 24*E + 2n + 4
+    per automaton over the kernel time measured with HIP events on the
+    solver's stream; `traffic` from the committed rocprofv3 PMC summary;
+  * cpu_baseline: the C restatement of the reference's SlimCPUUntweaked
+    search (oracle/ssw_oracle.c) on a bounded sample, on the host cores;
+  * the other workloads, each run once: plain_n10 (10k automata, n = 10, with
+    witnesses re-verified on Q), uneven_12_24 (200k automata of uneven n in
+    one call), large (Cerny C_24..C_30 + 64 random n = 30, device-wide path)
+    and serving (Poisson stream, n in 8..30, p50/p99 latency).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extras]
 """
 from __future__ import annotations
 
@@ -29,23 +39,30 @@ import workloads  # noqa: E402
 
 METRIC = "automata/s + subsets/s at n=16; p99 latency; batched/served vs plain GPU BFS"
 HBM_PEAK_GBS = 8000.0
+TUNING = os.path.join(PKG, "tuning_gfx950.json")
+TRAFFIC = os.path.join(ROOT, "profiles", "batch_kernel_uniform_n16_traffic.txt")
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def algorithmic_bytes(n, expanded):
-    """Bytes the search must move per automaton: 2n transition bytes, per
-    expanded subset a 4-byte frontier read and two 8-byte test-and-set RMWs,
-    a 4-byte frontier write per discovered non-root subset, 8 result bytes:
-    24*E + 2n + 4."""
-    E = expanded.astype(np.float64)
+    """Per automaton: 2n transition bytes; per expanded subset a 4-byte
+    frontier read and two 8-byte test-and-set read-modify-writes; 4 bytes per
+    discovered non-root subset written to the frontier; 8 result bytes.
+    Total 24*E + 2n + 4."""
+    E = np.asarray(expanded, np.float64)
+    n = np.asarray(n, np.float64)
     return float(np.sum(24.0 * E + 2.0 * n + 4.0))
 
 
 def traffic_from_profile():
-    """HBM bytes per launch measured by rocprofv3 PMC runs (committed summary)."""
-    path = os.path.join(ROOT, "profiles", "batch_kernel_uniform_n16_traffic.txt")
-    if not os.path.exists(path):
+    """HBM bytes per launch of the batched kernel, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by a copy kernel)."""
+    if not os.path.exists(TRAFFIC):
         return None
-    for line in open(path):
+    for line in open(TRAFFIC):
         if line.startswith("traffic_bytes_per_launch:"):
             try:
                 return float(line.split(":", 1)[1].strip())
@@ -59,67 +76,179 @@ def cpu_baseline(n, delta, sample, threads):
     import oracle
     A = delta[:sample, :n].astype(np.int32)
     B = delta[:sample, n:].astype(np.int32)
-    oracle.slim(n, A[:1000], B[:1000], threads=threads)  # warm-up
+    oracle.slim(n, A[:2000], B[:2000], threads=threads)  # warm-up
     t0 = time.perf_counter()
     L, S, E, _ = oracle.slim(n, A, B, threads=threads)
     dt = time.perf_counter() - t0
     return sample / dt, L, S, E, dt
 
 
+def verify_words(n, A, B, L, wit):
+    """Re-applies each witness to every state (Verify.cs:12-53), vectorised."""
+    ok = True
+    idx = np.nonzero(L >= 0)[0]
+    for i in idx:
+        w = ssw.witness_bits_to_word(wit[i], int(L[i]))
+        s = np.arange(n)
+        for x in w:
+            s = (B[i] if x else A[i])[s]
+        ok &= bool(np.all(s == s[0]))
+    return ok
+
+
+def bench_plain_n10(ctx):
+    n, A, B = workloads.plain_n10()
+    ctx.compute(n, A[:100], B[:100], witness=True)
+    t0 = time.perf_counter()
+    r = ctx.compute(n, A, B, witness=True)
+    dt = time.perf_counter() - t0
+    return {"workload": "10,000 random automata n=10 (seed 1001), lengths + witnesses, host buffers (PCIe included)",
+            "automata_per_s": len(A) / dt, "ms": dt * 1e3,
+            "non_synchronizing": int((r.length < 0).sum()), "max_length": int(r.length.max()),
+            "witnesses_verified_on_Q": verify_words(n, A, B, r.length, r.witness)}
+
+
+def bench_uneven(ctx):
+    npr, offs, delta = workloads.uneven_12_24()
+    count = len(npr)
+    d = [ctx.to_device(x) for x in (delta, npr, offs)]
+    d_len, d_exp = ctx.alloc(4 * count), ctx.alloc(8 * count)
+    b = ssw.Batch(count=count, uniform_n=0, witness_stride=0, n_per=d[1], offset=d[2], delta=d[0],
+                  length=d_len, sync=None, expanded=d_exp, witness=None, bound=-1, reserved=0)
+    ctx.solve_device(b)
+    ts = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        ctx.solve_device(b)
+        ts.append(time.perf_counter() - t0)
+    E = ctx.to_host(d_exp, np.int64, count)
+    L = ctx.to_host(d_len, np.int32, count)
+    dt = min(ts)
+    for p in d + [d_len, d_exp]:
+        ctx.free(p)
+    return {"workload": "200,000 random automata, n in 12..24 with P(n) ~ 2^-(n-12)/2 (seed 1003), one call",
+            "automata_per_s": count / dt, "subsets_per_s": float(E.sum()) / dt, "ms": dt * 1e3,
+            "launches": ctx.last_kernel_time()[1], "counts_per_n": np.bincount(npr)[12:].tolist(),
+            "non_synchronizing": int((L < 0).sum())}
+
+
+def bench_large(ctx):
+    items = workloads.large()
+    ns = [it[0] for it in items]
+    t0 = time.perf_counter()
+    r = ctx.solve_mixed(ns, [it[1] for it in items], [it[2] for it in items], mode="large")
+    dt = time.perf_counter() - t0
+    cerny_ok = all(int(r.length[i]) == (n - 1) ** 2 for i, (n, a, b, k) in enumerate(items) if k == "cerny")
+    return {"workload": "Cerny C_24..C_30 + 64 random n=30 (seed 1004), all concurrent, device-wide path",
+            "wall_s": dt, "subsets": int(r.expanded.sum()), "subsets_per_s": float(r.expanded.sum()) / dt,
+            "cerny_lengths_equal_(n-1)^2": cerny_ok,
+            "cerny_expanded_equal_2^n-n-1": all(int(r.expanded[i]) == 2 ** n - n - 1
+                                               for i, (n, a, b, k) in enumerate(items) if k == "cerny"),
+            "random_n30_lengths": sorted(set(int(x) for x, it in zip(r.length, items) if it[3] == "random"))}
+
+
+def bench_serving(ctx, rate, count):
+    reqs = workloads.serving(rate=rate, count=count)
+    with ssw.Server(ctx, small_n_max=20, levels_per_tick=16) as srv:
+        # warm both engines
+        a, b = ssw.generate_cerny(12, 1)
+        srv.wait(srv.submit(12, a, b))
+        a, b = ssw.generate_cerny(22, 1)
+        srv.wait(srv.submit(22, a, b))
+    with ssw.Server(ctx, small_n_max=20, levels_per_tick=16) as srv:
+        t0 = time.perf_counter()
+        tickets = []
+        for (t, n, a, b, kind) in reqs:
+            now = time.perf_counter() - t0
+            if now < t:
+                time.sleep(t - now)
+            tickets.append(srv.submit(n, a, b))
+        res = [srv.wait(tk, 600000) for tk in tickets]
+        st = srv.stats()
+    lat = np.array([r["latency_us"] for r in res])
+    cer = np.array([k == "cerny" for (_, _, _, _, k) in reqs])
+    ok = all(r["length"] == (n - 1) ** 2 for r, (t, n, a, b, k) in zip(res, reqs) if k == "cerny")
+    return {"workload": f"Poisson stream {rate:g} req/s, {count} requests, n uniform in 8..30, 20% Cerny (seed 1005)",
+            "p50_ms": st["p50_us"] / 1e3, "p99_ms": st["p99_us"] / 1e3, "max_ms": st["max_us"] / 1e3,
+            "random_p99_ms": float(np.percentile(lat[~cer], 99)) / 1e3 if (~cer).any() else None,
+            "automata_per_s": st["automata_per_s"], "subsets_per_s":
+                st["subsets_expanded"] / max(1e-9, (reqs[-1][0] + st["max_us"] / 1e6)),
+            "completed": st["completed"], "cerny_lengths_ok": ok}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--count", type=int, default=1_000_000)
     ap.add_argument("--cpu-sample", type=int, default=250_000)
     ap.add_argument("--plain-sample", type=int, default=5_000)
+    ap.add_argument("--serving-rate", type=float, default=200.0)
+    ap.add_argument("--serving-count", type=int, default=600)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="also time a device-to-device copy of this size (PMC calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: every automaton is independent, so N GPUs would each "
-              "solve their own shard of the batch; run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the automata are independent, so on N GPUs each would solve "
+              "its own shard of the batch with no communication; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     n, delta, pos = workloads.uniform_n16(count=args.count)
     count = len(delta)
     ctx = ssw.Context(0)
+    if os.path.exists(TUNING):
+        ctx.load_tuning(TUNING)
     d_delta = ctx.to_device(delta.reshape(-1))
     d_len = ctx.alloc(4 * count)
     d_sync = ctx.alloc(count)
     d_exp = ctx.alloc(8 * count)
     batch = ssw.Batch(count=count, uniform_n=n, witness_stride=0, n_per=None, offset=None, delta=d_delta,
                       length=d_len, sync=d_sync, expanded=d_exp, witness=None, bound=-1, reserved=0)
+    if not args.no_tune:
+        t0 = time.perf_counter()
+        ctx.autotune_batch(batch)
+        log(f"autotuned on the batch in {time.perf_counter() - t0:.2f} s: {ctx.get_tuning(n)}")
 
     for _ in range(args.warmup):
         ctx.solve_device(batch)
-    ctx.solve_device(batch)
-    kernel_ms, launches = ctx.last_kernel_time()
     ctx.synchronize()
-    t0 = time.perf_counter()
     kms = []
+    t0 = time.perf_counter()
     for _ in range(args.steps):
         ctx.solve_device(batch)
         kms.append(ctx.last_kernel_time()[0])
     ctx.synchronize()
     dt = time.perf_counter() - t0
     ms_per_step = 1e3 * dt / args.steps
+    launches = ctx.last_kernel_time()[1]
     length = ctx.to_host(d_len, np.int32, count)
     sync = ctx.to_host(d_sync, np.uint8, count)
     expanded = ctx.to_host(d_exp, np.int64, count)
     subsets = float(expanded.sum())
     kernel_ms = float(np.mean(kms))
+    log(f"step {ms_per_step:.3f} ms, kernel {kernel_ms:.3f} ms, {count / (ms_per_step * 1e-3):.3e} automata/s")
+
+    if args.copy_bytes:
+        src, dst = ctx.alloc(args.copy_bytes), ctx.alloc(args.copy_bytes)
+        ctx.copy_on_device(dst, src, args.copy_bytes)
+        ctx.free(src)
+        ctx.free(dst)
 
     # plain one-automaton-per-launch GPU BFS on a sample with the same Cerny share
     ps = min(args.plain_sample, count)
     pdelta = np.ascontiguousarray(delta[:ps])
+    ctx.solve_packed(pdelta[:50].reshape(-1), 50, n, mode="plain")
     t1 = time.perf_counter()
     pr = ctx.solve_packed(pdelta.reshape(-1), ps, n, mode="plain")
     plain_dt = time.perf_counter() - t1
     plain_rate = ps / plain_dt
     assert np.array_equal(pr.length, length[:ps]), "plain and batched paths disagree"
 
-    abytes = algorithmic_bytes(n, expanded)
+    abytes = algorithmic_bytes(np.full(count, n), expanded)
     achieved = abytes / (kernel_ms * 1e-3) / 1e9
     traffic = traffic_from_profile()
 
@@ -132,8 +261,10 @@ def main():
         parity = bool(np.array_equal(L, length[:sample]) and np.array_equal(S, sync[:sample])
                       and np.array_equal(E, expanded[:sample]))
         cpu = {"value": rate, "unit": "automata/s", "cores": threads, "kind": "port",
-               "sample": f"first {sample} automata of the same batch (1% Cerny C_16), "
-                         f"oracle/ssw_oracle.c restatement of SlimCPUUntweaked, {cdt:.2f} s wall"}
+               "sample": f"first {sample} automata of the same batch ({int((pos < sample).sum())} Cerny C_16), "
+                         f"oracle/ssw_oracle.c restatement of SlimCPUUntweaked on {threads} threads, "
+                         f"{cdt:.2f} s wall"}
+        log(f"cpu baseline {rate:.3e} automata/s on {threads} threads, parity {parity}")
 
     value = count / (ms_per_step * 1e-3)
     out = {
@@ -150,8 +281,9 @@ def main():
         "dtype": "u32",
         "data": "synthetic",
         "config": {"workload": "uniform_n16: 1,000,000 random binary automata n=16 (splitmix64 seed 1002), "
-                               "1% replaced by Cerny C_16; lengths+flags+expanded counts, no witness",
-                   "count": count, "n": n, "cerny": int(len(pos)), "launches_per_step": int(launches)},
+                               "1% replaced by Cerny C_16; lengths + flags + expanded counts, inputs in HBM",
+                   "count": count, "n": n, "cerny": int(len(pos)), "launches_per_step": int(launches),
+                   "tuning": ctx.get_tuning(n)},
         "subsets_per_s": subsets / (ms_per_step * 1e-3),
         "subsets_per_step": subsets,
         "kernel_ms": kernel_ms,
@@ -163,7 +295,14 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out))
+    if not args.no_extras:
+        for name, fn in (("plain_n10", lambda: bench_plain_n10(ctx)), ("uneven_12_24", lambda: bench_uneven(ctx)),
+                         ("large", lambda: bench_large(ctx)),
+                         ("serving", lambda: bench_serving(ctx, args.serving_rate, args.serving_count))):
+            t0 = time.perf_counter()
+            out[name] = fn()
+            log(f"{name} done in {time.perf_counter() - t0:.1f} s")
+    print(json.dumps(out), flush=True)
     ctx.close()
 
 

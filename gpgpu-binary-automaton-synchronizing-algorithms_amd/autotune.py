@@ -1,0 +1,40 @@
+"""Runs the library's per-n autotuner on device 0 and writes the tuning table.
+
+    python autotune.py [--lo 4] [--hi 24] [--samples 200000] [--out tuning_gfx950.json]
+
+bench.py and the tests load the committed table (tuning_gfx950.json) when it
+exists; without it the library's built-in defaults apply.
+"""
+import argparse
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+import ssw  # noqa: E402
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--lo", type=int, default=4)
+    ap.add_argument("--hi", type=int, default=24)
+    ap.add_argument("--samples", type=int, default=200_000)
+    ap.add_argument("--seed", type=int, default=77)
+    ap.add_argument("--out", default=os.path.join(HERE, "tuning_gfx950.json"))
+    args = ap.parse_args()
+    with ssw.Context(0) as ctx:
+        for n in range(args.lo, args.hi + 1):
+            t0 = time.perf_counter()
+            ctx.autotune(n, n, args.samples, args.seed, None)
+            print(f"n={n}: {ctx.get_tuning(n)} ({time.perf_counter() - t0:.1f} s)", flush=True)
+        import json
+        table = {"device": "gfx950", "samples": args.samples,
+                 "tuning": {str(n): ctx.get_tuning(n) for n in range(args.lo, args.hi + 1)}}
+        with open(args.out, "w") as f:
+            json.dump(table, f, indent=1)
+        print("wrote", args.out)
+
+
+if __name__ == "__main__":
+    main()

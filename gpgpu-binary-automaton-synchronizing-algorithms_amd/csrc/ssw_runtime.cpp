@@ -783,6 +783,14 @@ int ssw_copy_to_host(ssw_context *ctx, void *dst, const void *src, uint64_t byte
     return e == hipSuccess ? SSW_OK : ctx->hip_fail(e, "copy to host");
 }
 
+int ssw_copy_on_device(ssw_context *ctx, void *dst, const void *src, uint64_t bytes) {
+    CTX_GUARD(ctx);
+    hipSetDevice(ctx->device);
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    return e == hipSuccess ? SSW_OK : ctx->hip_fail(e, "copy on device");
+}
+
 int ssw_synchronize(ssw_context *ctx) {
     CTX_GUARD(ctx);
     hipSetDevice(ctx->device);

@@ -25,9 +25,10 @@ LEN_INVALID = -3
 EXPORTS = [
     "ssw_abi_version", "ssw_create", "ssw_destroy", "ssw_last_error", "ssw_compute", "ssw_verify",
     "ssw_best_parallelism", "ssw_solve_batch", "ssw_solve_plain", "ssw_solve_large",
-    "ssw_get_tuning", "ssw_set_tuning", "ssw_autotune", "ssw_load_tuning", "ssw_get_progress",
+    "ssw_get_tuning", "ssw_set_tuning", "ssw_autotune", "ssw_autotune_batch", "ssw_load_tuning",
+    "ssw_get_progress",
     "ssw_set_progress_callback", "ssw_device_alloc", "ssw_device_free", "ssw_copy_to_device",
-    "ssw_copy_to_host", "ssw_synchronize", "ssw_last_kernel_time", "ssw_generate_uniform",
+    "ssw_copy_to_host", "ssw_copy_on_device", "ssw_synchronize", "ssw_last_kernel_time", "ssw_generate_uniform",
     "ssw_generate_cerny", "ssw_generate_dotnet", "ssw_server_create", "ssw_server_submit",
     "ssw_server_poll", "ssw_server_wait", "ssw_server_stats_get", "ssw_server_destroy",
 ]
@@ -113,12 +114,14 @@ def lib():
             "ssw_set_tuning": ([vp, i32, ctypes.POINTER(Tuning)], ctypes.c_int),
             "ssw_autotune": ([vp, i32, i32, i64, u64, ctypes.c_char_p], ctypes.c_int),
             "ssw_load_tuning": ([vp, ctypes.c_char_p], ctypes.c_int),
+            "ssw_autotune_batch": ([vp, ctypes.POINTER(Batch), i32], ctypes.c_int),
             "ssw_get_progress": ([vp, ctypes.POINTER(Progress)], ctypes.c_int),
             "ssw_set_progress_callback": ([vp, PROGRESS_FN, vp, i32], ctypes.c_int),
             "ssw_device_alloc": ([vp, u64, ctypes.POINTER(vp)], ctypes.c_int),
             "ssw_device_free": ([vp, vp], ctypes.c_int),
             "ssw_copy_to_device": ([vp, vp, vp, u64], ctypes.c_int),
             "ssw_copy_to_host": ([vp, vp, vp, u64], ctypes.c_int),
+            "ssw_copy_on_device": ([vp, vp, vp, u64], ctypes.c_int),
             "ssw_synchronize": ([vp], ctypes.c_int),
             "ssw_last_kernel_time": ([vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i64)], ctypes.c_int),
             "ssw_generate_uniform": ([u64, i32, i64, i64, vp, vp], ctypes.c_int),
@@ -329,6 +332,9 @@ class Context:
         self._check(self._L.ssw_copy_to_host(self.h, _ptr(out), p, out.nbytes), "copy")
         return out
 
+    def copy_on_device(self, dst, src, nbytes):
+        self._check(self._L.ssw_copy_on_device(self.h, dst, src, nbytes), "copy on device")
+
     def solve_device(self, batch: Batch, mode="batch"):
         viol = ctypes.c_int64(-1)
         fn = {"batch": self._L.ssw_solve_batch, "plain": self._L.ssw_solve_plain,
@@ -359,6 +365,9 @@ class Context:
     def autotune(self, n_lo, n_hi, samples=20000, seed=7, out_path=None):
         self._check(self._L.ssw_autotune(self.h, n_lo, n_hi, samples, seed,
                                          out_path.encode() if out_path else None), "autotune")
+
+    def autotune_batch(self, batch: Batch):
+        self._check(self._L.ssw_autotune_batch(self.h, ctypes.byref(batch), SSW_DEVICE), "autotune_batch")
 
     def load_tuning(self, path):
         self._check(self._L.ssw_load_tuning(self.h, path.encode()), "load_tuning")

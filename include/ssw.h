@@ -156,6 +156,9 @@ int ssw_set_tuning(ssw_context *ctx, int32_t n, const ssw_tuning *t);
 int ssw_autotune(ssw_context *ctx, int32_t n_lo, int32_t n_hi, int64_t samples,
                  uint64_t seed, const char *out_path);
 int ssw_load_tuning(ssw_context *ctx, const char *path);
+/* Tunes the entry of n = b->uniform_n on the caller's own device batch (for
+   instance a workload with long-tail automata); memory must be SSW_DEVICE. */
+int ssw_autotune_batch(ssw_context *ctx, const ssw_batch *b, int32_t memory);
 
 /* ---- progress (device counters) ----------------------------------------- */
 
@@ -181,6 +184,7 @@ int ssw_device_alloc(ssw_context *ctx, uint64_t bytes, void **out);
 int ssw_device_free(ssw_context *ctx, void *p);
 int ssw_copy_to_device(ssw_context *ctx, void *dst, const void *src, uint64_t bytes);
 int ssw_copy_to_host(ssw_context *ctx, void *dst, const void *src, uint64_t bytes);
+int ssw_copy_on_device(ssw_context *ctx, void *dst, const void *src, uint64_t bytes);
 int ssw_synchronize(ssw_context *ctx);
 /* Average duration (ms) of the last solve's main kernel, measured with HIP
    events on the context's stream, and its launch count. */
