@@ -188,7 +188,8 @@ def main():
     ap.add_argument("--serving-count", type=int, default=600)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
-    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--tune", action="store_true",
+                    help="autotune n=16 on this batch before timing (default: the committed table)")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="also time a device-to-device copy of this size (PMC calibration)")
     args = ap.parse_args()
@@ -208,7 +209,7 @@ def main():
     d_exp = ctx.alloc(8 * count)
     batch = ssw.Batch(count=count, uniform_n=n, witness_stride=0, n_per=None, offset=None, delta=d_delta,
                       length=d_len, sync=d_sync, expanded=d_exp, witness=None, bound=-1, reserved=0)
-    if not args.no_tune:
+    if args.tune:
         t0 = time.perf_counter()
         ctx.autotune_batch(batch)
         log(f"autotuned on the batch in {time.perf_counter() - t0:.2f} s: {ctx.get_tuning(n)}")
