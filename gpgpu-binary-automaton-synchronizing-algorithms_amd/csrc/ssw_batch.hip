@@ -373,6 +373,7 @@ struct GroupSolver {
             bad = true;
         if (g.any(bad)) {
             if (r == 0) write_invalid();
+            if (P.witness) clear_witness_row();
             return false;
         }
         const uint8_t *d = P.delta + (P.offset ? P.offset[ai] : (uint64_t)ai * 2u * (uint64_t)n);
@@ -381,6 +382,7 @@ struct GroupSolver {
         for (int q = (int)r; q < 2 * n; q += (int)G) badd |= d[q] >= n;
         if (g.any(badd)) {
             if (r == 0) write_invalid();
+            if (P.witness) clear_witness_row();
             return false;
         }
         Q = (1u << n) - 1u;
@@ -433,6 +435,12 @@ struct GroupSolver {
         P.length[ai] = -3;
         if (P.sync) P.sync[ai] = 0;
         if (P.expanded) P.expanded[ai] = 0;
+    }
+
+    // an automaton without a reset word gets an all-zero witness row
+    __device__ void clear_witness_row() const {
+        glb_u8 *out = (glb_u8 *)(P.witness + (uint64_t)ai * (uint64_t)P.witness_stride);
+        for (int32_t k = (int32_t)g.rank(); k < P.witness_stride; k += (int32_t)G) out[k] = 0;
     }
 
     // One level step over up to G frontier entries.
@@ -558,6 +566,7 @@ struct GroupSolver {
                 if (found && L > bound) atomicMin(P.violation, (unsigned long long)ai);
             }
             if (found && P.witness) witness(L);
+            if (!found && P.witness) clear_witness_row();
             ++done_local;
             subsets_local += (uint64_t)E;
         }

@@ -332,8 +332,12 @@ int LargeEngine::solve_indices(ssw_context &ctx, const DeviceBatch &db, const ui
                 clear_slot(ctx, s);
                 if (I.length == -3) {
                     // invalid transitions: report like the batched kernel
-                    int32_t v = -3;
+                    static const int32_t v = -3;
                     LG_TRY(hipMemcpyAsync(db.length + bad, &v, 4, hipMemcpyHostToDevice, ctx.stream), "H2D");
+                    if (append)
+                        LG_TRY(hipMemsetAsync(db.witness + (uint64_t)bad * db.witness_stride, 0, db.witness_stride,
+                                              ctx.stream),
+                               "clear witness");
                     continue;
                 }
                 return ctx.fail(SSW_ERR_INTERNAL, "device-wide path: frontier overflow on automaton %lld",
