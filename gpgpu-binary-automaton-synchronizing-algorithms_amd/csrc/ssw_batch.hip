@@ -767,31 +767,41 @@ struct SmallSolver {
 
     __device__ int step() {
         if (in_lds && wr + 2u * G * T - rd > P.lds_arena_cap) migrate();
-        // one code path per ring location, so the LDS path carries no wait
-        // on outstanding HBM loads (the prefetched next automaton)
-        return in_lds ? step_in<true>() : step_in<false>();
+        // entries per lane follow the width left in the level (wide C_n levels
+        // take T per lane, narrow random-automaton levels one); one code path
+        // per ring location, so the LDS path carries no wait on outstanding
+        // HBM loads (the prefetched next automaton)
+        const uint32_t left = hi - rd;
+        if constexpr (T >= 4) {
+            if (left >= 4u * G) return in_lds ? step_in<true, 4>() : step_in<false, 4>();
+        }
+        if constexpr (T >= 2) {
+            if (left >= 2u * G) return in_lds ? step_in<true, 2>() : step_in<false, 2>();
+        }
+        return in_lds ? step_in<true, 1>() : step_in<false, 1>();
     }
 
-    template <bool LDS>
+    template <bool LDS, int TT>
     __device__ int step_in() {
+        constexpr int kBitsTT = (2 * TT >= 8) ? 4 : (2 * TT >= 4 ? 3 : 2);
         const uint32_t r = g.rank();
-        uint32_t s[T];
-        bool v[T];
+        uint32_t s[TT];
+        bool v[TT];
 #pragma unroll
-        for (int t = 0; t < T; ++t) {
+        for (int t = 0; t < TT; ++t) {
             const uint32_t p = rd + t * G + r;
             v[t] = p < hi;
             // lanes past the level read a harmless slot and are masked below
             s[t] = rget<LDS>(v[t] ? p : rd);
             s[t] = v[t] ? s[t] : 0u;
         }
-        uint32_t img[T];
+        uint32_t img[TT];
 #pragma unroll
-        for (int t = 0; t < T; ++t) img[t] = image(s[t]);
+        for (int t = 0; t < TT; ++t) img[t] = image(s[t]);
         bool single = false;
         uint32_t c = 0, newmask = 0;
 #pragma unroll
-        for (int t = 0; t < T; ++t) {
+        for (int t = 0; t < TT; ++t) {
             const uint32_t a = img[t] & 0xFFFFu, b = img[t] >> 16;
             const bool sa = v[t] && is_single(a) && a != Q;
             const bool sb = v[t] && is_single(b) && b != Q;
@@ -814,16 +824,16 @@ struct SmallSolver {
         }
         bool any_single;
         uint32_t prefix, total;
-        g.template exchange_count<kBits>(single, c, any_single, prefix, total);
+        g.template exchange_count<kBitsTT>(single, c, any_single, prefix, total);
         uint32_t k = wr + prefix;
 #pragma unroll
-        for (int t = 0; t < T; ++t) {
+        for (int t = 0; t < TT; ++t) {
             if ((newmask >> (2 * t)) & 1u) rput<LDS>(k++, img[t] & 0xFFFFu);
             if ((newmask >> (2 * t + 1)) & 1u) rput<LDS>(k++, img[t] >> 16);
         }
         wr = uni<G>(wr + total);
         if (uni<G>(any_single ? 1u : 0u)) return kFound;
-        rd = uni<G>(rd + G * T);
+        rd = uni<G>(rd + G * TT);
         if (rd >= hi) {
             if (G > 64 || !LDS)
                 g.sync();

@@ -8,6 +8,7 @@
 #include "ssw_host.h"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 
 namespace ssw {
@@ -73,7 +74,12 @@ int LargeEngine::setup(ssw_context &ctx) {
         slots.assign(kLargeMaxInstances, LargeSlot());
         hinst.assign(kLargeMaxInstances, LargeInstance());
         num = 0;
-        grid = std::max(1, ctx.cus) * 4;
+        // workgroups of large_step per CU (SSW_LARGE_GRID_PER_CU overrides the
+        // default 1: more workgroups only add contention on the bitmaps)
+        int per_cu = 1;
+        if (const char *e = getenv("SSW_LARGE_GRID_PER_CU")) per_cu = std::max(1, std::min(32, atoi(e)));
+        grid = std::max(1, ctx.cus) * per_cu;
+        if (const char *e = getenv("SSW_LARGE_LEVELS")) tick_levels = std::max(1, std::min(256, atoi(e)));
     }
     return SSW_OK;
 }
