@@ -54,6 +54,8 @@ struct ClassSpec {
     const uint32_t *order = nullptr;
     const uint8_t *host_n = nullptr;  // plain path: per-automaton n on the host
     int plain_n = 0;
+    hipStream_t stream = nullptr;     // null: the context's stream
+    uint32_t *queues = nullptr;       // this class's device work queue
 };
 
 struct SlotKey {
@@ -135,6 +137,10 @@ struct ssw_context {
     hipStream_t stream = nullptr;
     hipStream_t monitor = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    static constexpr int kClassStreams = 8;   // concurrent launch classes of one call
+    hipStream_t class_streams[kClassStreams] = {};
+    hipEvent_t ev_fork = nullptr;
+    hipEvent_t ev_join[kClassStreams] = {};
     uint64_t budget = 0;
     std::mutex mu;           // serializes calls on the context
     std::mutex monitor_mu;   // progress reads
@@ -143,6 +149,8 @@ struct ssw_context {
     ssw::DevBuf ctrs, queues, deferred, order;
     ssw::DevBuf d_delta, d_nper, d_off, d_len, d_sync, d_exp, d_wit;
     std::map<ssw::SlotKey, ssw::DevBuf> slot_cache;
+    std::map<ssw::SlotKey, uint64_t> slot_gen;   // call that last used each entry
+    uint64_t call_gen = 0;
     ssw::LargeEngine large;
     ssw_tuning tune[SSW_MAX_N + 1];
 
@@ -168,7 +176,7 @@ struct ssw_context {
     int fail(int code, const char *fmt, ...);
     int hip_fail(hipError_t e, const char *what);
     int ensure(ssw::DevBuf &b, size_t bytes, const char *what);
-    int get_slots(const ssw::SlotKey &key, uint64_t bytes, ssw::DevBuf **out);
+    int get_slots(const ssw::SlotKey &key, uint64_t bytes, ssw::DevBuf **out, hipStream_t st);
     int launch_class(const ssw::DeviceBatch &db, const ssw::ClassSpec &cls, bool plain);
     int solve_device(const ssw::DeviceBatch &db, const uint8_t *host_n, bool plain, int64_t *violation_out);
     int solve_host_packed(const ssw_batch *b, int mode, int64_t *violation_out);

@@ -94,6 +94,11 @@ int ssw_context::open(int dev, uint64_t mem_budget) {
     HIP_TRY(hipStreamCreateWithFlags(&monitor, hipStreamNonBlocking), "hipStreamCreate");
     HIP_TRY(hipEventCreate(&ev0), "hipEventCreate");
     HIP_TRY(hipEventCreate(&ev1), "hipEventCreate");
+    HIP_TRY(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming), "hipEventCreate");
+    for (int k = 0; k < kClassStreams; ++k) {
+        HIP_TRY(hipStreamCreateWithFlags(&class_streams[k], hipStreamNonBlocking), "hipStreamCreate");
+        HIP_TRY(hipEventCreateWithFlags(&ev_join[k], hipEventDisableTiming), "hipEventCreate");
+    }
     size_t free_b = 0, total_b = 0;
     HIP_TRY(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
     budget = mem_budget ? mem_budget : (uint64_t)(0.8 * (double)free_b);
@@ -119,6 +124,11 @@ void ssw_context::close() {
     large.release();
     if (ev0) hipEventDestroy(ev0);
     if (ev1) hipEventDestroy(ev1);
+    if (ev_fork) hipEventDestroy(ev_fork);
+    for (int k = 0; k < kClassStreams; ++k) {
+        if (class_streams[k]) hipStreamDestroy(class_streams[k]);
+        if (ev_join[k]) hipEventDestroy(ev_join[k]);
+    }
     if (stream) hipStreamDestroy(stream);
     if (monitor) hipStreamDestroy(monitor);
 }
@@ -144,9 +154,10 @@ void ssw_context::default_tuning() {
 // ---------------------------------------------------------------------------
 
 // Per-group scratch in HBM for one launch configuration.
-int ssw_context::get_slots(const SlotKey &key, uint64_t bytes, DevBuf **out) {
+int ssw_context::get_slots(const SlotKey &key, uint64_t bytes, DevBuf **out, hipStream_t st) {
     auto it = slot_cache.find(key);
     if (it != slot_cache.end() && it->second.bytes >= bytes) {
+        slot_gen[key] = call_gen;
         *out = &it->second;
         return SSW_OK;
     }
@@ -156,19 +167,24 @@ int ssw_context::get_slots(const SlotKey &key, uint64_t bytes, DevBuf **out) {
     }
     uint64_t used = 0;
     for (auto &kv : slot_cache) used += kv.second.bytes;
-    // evict other configurations rather than exceed the budget
-    while (!slot_cache.empty() && used + bytes > budget) {
-        auto victim = slot_cache.begin();
-        used -= victim->second.bytes;
-        hipFree(victim->second.p);
-        slot_cache.erase(victim);
+    // evict configurations not used by this call rather than exceed the budget
+    for (auto v = slot_cache.begin(); v != slot_cache.end() && used + bytes > budget;) {
+        if (slot_gen[v->first] == call_gen) {
+            ++v;
+            continue;
+        }
+        used -= v->second.bytes;
+        hipFree(v->second.p);
+        slot_gen.erase(v->first);
+        v = slot_cache.erase(v);
     }
     DevBuf b;
     int rc = ensure(b, bytes, "slot scratch");
     if (rc) return rc;
     // bitmaps must start clean; the kernels keep them clean afterwards
-    HIP_TRY(hipMemsetAsync(b.p, 0, b.bytes, stream), "hipMemset slots");
+    HIP_TRY(hipMemsetAsync(b.p, 0, b.bytes, st), "hipMemset slots");
     slot_cache[key] = b;
+    slot_gen[key] = call_gen;
     *out = &slot_cache[key];
     return SSW_OK;
 }
@@ -209,7 +225,8 @@ int ssw_context::launch_class(const DeviceBatch &db, const ClassSpec &cls, bool 
     P.witness_stride = db.witness_stride;
     P.want_witness = want_wit ? 1u : 0u;
     uint8_t *c = (uint8_t *)ctrs.p;
-    P.queues = (uint32_t *)queues.p;
+    P.queues = cls.queues;
+    hipStream_t st = cls.stream ? cls.stream : stream;
     P.deferred_count = (unsigned int *)(c + kCtrDeferred);
     P.violation = (unsigned long long *)(c + kCtrViolation);
     P.progress = (unsigned long long *)(c + kCtrProgress);
@@ -305,7 +322,7 @@ int ssw_context::launch_class(const DeviceBatch &db, const ClassSpec &cls, bool 
     const uint64_t slots = (uint64_t)grid * groups_per_block;
     SlotKey key{G, block, P.lds_n, nmax, (int)want_wit, (int)arena_lds, (plain ? 1 : 0) + 2 * variant};
     DevBuf *sb = nullptr;
-    int rc = get_slots(key, slots * P.slot_bytes, &sb);
+    int rc = get_slots(key, slots * P.slot_bytes, &sb, st);
     if (rc) return rc;
     P.slot_base = (uint8_t *)sb->p;
     last_grid = grid;
@@ -314,15 +331,15 @@ int ssw_context::launch_class(const DeviceBatch &db, const ClassSpec &cls, bool 
     last_variant = variant;
 
     if (!plain) {
-        HIP_TRY(init_queues(P.queues, P.count, stream), "init queues");
-        HIP_TRY(launch_batch(variant, G, P, grid, block, lds, stream), "batch kernel launch");
+        HIP_TRY(init_queues(P.queues, P.count, st), "init queues");
+        HIP_TRY(launch_batch(variant, G, P, grid, block, lds, st), "batch kernel launch");
         ++launches;
     } else {
         for (int64_t i = 0; i < db.count; ++i) {
             if (cls.host_n && cls.host_n[i] != cls.plain_n) continue;
             P.single_item = 1;
             P.first_item = i;
-            HIP_TRY(launch_batch(variant, G, P, 1, block, lds, stream), "plain kernel launch");
+            HIP_TRY(launch_batch(variant, G, P, 1, block, lds, st), "plain kernel launch");
             ++launches;
         }
     }
@@ -334,6 +351,7 @@ int ssw_context::launch_class(const DeviceBatch &db, const ClassSpec &cls, bool 
 int ssw_context::solve_device(const DeviceBatch &db, const uint8_t *host_n, bool plain,
                               int64_t *violation_out) {
     launches = 0;
+    ++call_gen;
     int rc = ensure(deferred, 4ull * std::max<int64_t>(1, db.count), "deferred list");
     if (rc) return rc;
     // violation <- UINT64_MAX, deferred count <- 0
@@ -357,6 +375,7 @@ int ssw_context::solve_device(const DeviceBatch &db, const uint8_t *host_n, bool
         }
     }
 
+    std::vector<uint32_t> host_order;
     if (plain) {
         // one launch per automaton, with the LDS bitmap whenever n <= 16
         for (int n = 1; n <= SSW_MAX_N; ++n) {
@@ -372,42 +391,69 @@ int ssw_context::solve_device(const DeviceBatch &db, const uint8_t *host_n, bool
             if (rc) return rc;
         }
     } else {
-        // class key: group lanes and LDS-or-HBM bitmap; automata of a class
-        // are ordered largest n first so the long ones start early
-        std::map<std::pair<int, int>, std::vector<int>> classes;  // (G, lds) -> list of n
-        for (int n = 1; n <= SSW_MAX_N; ++n)
-            if (hist[n]) classes[{tune[n].group_lanes, n <= kMaxLdsN ? 1 : 0}].push_back(n);
+        // Launch classes: one per n for the LDS-bitmap sizes (each launch
+        // sizes its LDS for its own n, so small automata keep many groups
+        // resident), one per thread-group width for the HBM-bitmap sizes.
+        // All classes run concurrently on their own streams, each with its
+        // own device work queue; inside a class the work order puts the
+        // largest n first so the long searches start early.
+        std::map<std::tuple<int, int, int>, std::vector<int>> classes;  // (G or 0, lds, n or 0) -> sizes
+        for (int n = 1; n <= SSW_MAX_N; ++n) {
+            if (!hist[n]) continue;
+            const bool lds = n <= kMaxLdsN;
+            classes[std::make_tuple(lds ? 0 : tune[n].group_lanes, lds ? 1 : 0, lds ? n : 0)].push_back(n);
+        }
         const bool single_class = classes.size() == 1 && db.uniform_n > 0;
-        std::vector<uint32_t> host_order;
+        std::vector<int64_t> class_start;
         if (!single_class) {
             rc = ensure(order, 4ull * db.count, "order");
             if (rc) return rc;
-            host_order.reserve(db.count);
+            // counting sort of the automata into class order, n descending
+            std::vector<int64_t> next(SSW_MAX_N + 1, 0);
+            int64_t pos = 0;
+            for (auto &kv : classes) {
+                class_start.push_back(pos);
+                std::vector<int> ns = kv.second;
+                std::sort(ns.rbegin(), ns.rend());
+                for (int n : ns) {
+                    next[n] = pos;
+                    pos += hist[n];
+                }
+            }
+            host_order.resize((size_t)db.count);
+            for (int64_t i = 0; i < db.count; ++i)
+                host_order[(size_t)next[db.uniform_n > 0 ? db.uniform_n : host_n[i]]++] = (uint32_t)i;
+            HIP_TRY(hipMemcpyAsync(order.p, host_order.data(), 4ull * db.count, hipMemcpyHostToDevice, stream),
+                    "upload order");
+        } else {
+            class_start.push_back(0);
         }
+        rc = ensure(queues, kQueueBytes * classes.size(), "work queues");
+        if (rc) return rc;
+        const int nstreams = std::min<int>((int)classes.size(), kClassStreams);
+        HIP_TRY(hipEventRecord(ev_fork, stream), "hipEventRecord");
+        for (int k = 0; k < nstreams; ++k) HIP_TRY(hipStreamWaitEvent(class_streams[k], ev_fork, 0), "wait fork");
+        int k = 0;
         for (auto &kv : classes) {
             ClassSpec cls;
-            cls.lds = kv.first.second != 0;
+            cls.lds = std::get<1>(kv.first) != 0;
             cls.nmax = *std::max_element(kv.second.begin(), kv.second.end());
             cls.items = 0;
             for (int n : kv.second) cls.items += hist[n];
-            cls.order = nullptr;
+            cls.order = single_class ? nullptr : (const uint32_t *)order.p + class_start[k];
             cls.host_n = nullptr;
             cls.plain_n = 0;
-            if (!single_class) {
-                const size_t start = host_order.size();
-                std::vector<int> ns = kv.second;
-                std::sort(ns.rbegin(), ns.rend());
-                for (int n : ns)
-                    for (int64_t i = 0; i < db.count; ++i)
-                        if ((db.uniform_n > 0 ? db.uniform_n : host_n[i]) == n) host_order.push_back((uint32_t)i);
-                cls.order = (const uint32_t *)order.p + start;
-                HIP_TRY(hipMemcpyAsync((uint32_t *)order.p + start, host_order.data() + start,
-                                       4ull * (host_order.size() - start), hipMemcpyHostToDevice, stream),
-                        "upload order");
-            }
+            cls.stream = nstreams > 1 ? class_streams[k % nstreams] : stream;
+            cls.queues = (uint32_t *)((uint8_t *)queues.p + kQueueBytes * k);
             rc = launch_class(db, cls, false);
             if (rc) return rc;
+            ++k;
         }
+        if (nstreams > 1)
+            for (int j = 0; j < nstreams; ++j) {
+                HIP_TRY(hipEventRecord(ev_join[j], class_streams[j]), "hipEventRecord");
+                HIP_TRY(hipStreamWaitEvent(stream, ev_join[j], 0), "wait join");
+            }
     }
     HIP_TRY(hipEventRecord(ev1, stream), "hipEventRecord");
 
