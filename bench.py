@@ -147,6 +147,22 @@ def bench_large(ctx):
             "random_n30_lengths": sorted(set(int(x) for x, it in zip(r.length, items) if it[3] == "random"))}
 
 
+def bench_verify_n13(ctx, count=1 << 21):
+    """IComputable.Verify over 2^21 automata with n = 13 drawn by the
+    System.Random twin (seed 1234567 + count, Program.cs:59,94), timed end to
+    end from host arrays as Program.cs:137-143 times Verify."""
+    A, B = ssw.generate_dotnet(1234567 + count, 13, count)
+    ctx.verify(13, A[:4096], B[:4096])
+    t0 = time.perf_counter()
+    bad = ctx.verify(13, A, B)
+    dt = time.perf_counter() - t0
+    return {"workload": f"Verify of {count} random automata n=13 (System.Random twin, seed 1234567+count), "
+                        "host arrays in, violating index out (PCIe included)",
+            "automata_verified_per_s": count / dt, "ms": dt * 1e3, "violation": bad,
+            "reference_published_n13": "SlimGPUQueue 1,953,926 automata/s at 2,097,152 problems "
+                                       "(GTX 1060, PDF p.18); not the headline metric"}
+
+
 def bench_serving(ctx, rate, count):
     reqs = workloads.serving(rate=rate, count=count)
     with ssw.Server(ctx, small_n_max=20, levels_per_tick=16) as srv:
@@ -298,6 +314,7 @@ def main():
     }
     if not args.no_extras:
         for name, fn in (("plain_n10", lambda: bench_plain_n10(ctx)), ("uneven_12_24", lambda: bench_uneven(ctx)),
+                         ("verify_n13", lambda: bench_verify_n13(ctx)),
                          ("large", lambda: bench_large(ctx)),
                          ("serving", lambda: bench_serving(ctx, args.serving_rate, args.serving_count))):
             t0 = time.perf_counter()
