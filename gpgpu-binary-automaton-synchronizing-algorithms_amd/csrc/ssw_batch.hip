@@ -115,6 +115,39 @@ struct WaveGroup {
         prefix = pre;
         total = tot;
     }
+    // any(single), and the exclusive prefix / total of the flags f[0..K) of
+    // every lane counted lane-major (all flags of lower lanes come first):
+    // one ballot per flag, the prefix from mbcnt
+    template <int K>
+    __device__ void exchange_flags(bool single, const bool (&f)[K], bool &any_single, uint32_t &prefix,
+                                   uint32_t &total) const {
+        any_single = (__ballot(single) & gmask) != 0;
+        uint32_t pre = 0, tot = 0;
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            const uint64_t m = __ballot(f[j]) & gmask;
+            pre = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, pre));
+            tot += (uint32_t)__popcll(m);
+        }
+        prefix = pre;
+        total = tot;
+    }
+    // any(single), and for every flag f[j] of every lane its position in a
+    // flag-major layout starting at base: all set f[0] of the group in lane
+    // order, then all set f[1], ...; total = number of set flags
+    template <int K>
+    __device__ void place(bool single, const bool (&f)[K], uint32_t base, bool &any_single, uint32_t (&pos)[K],
+                          uint32_t &total) const {
+        any_single = (__ballot(single) & gmask) != 0;
+        uint32_t b = base;
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            const uint64_t m = __ballot(f[j]) & gmask;
+            pos[j] = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, b));
+            b += (uint32_t)__popcll(m);
+        }
+        total = b - base;
+    }
     __device__ uint32_t bcast0(uint32_t v) const {
         if (G == 64) return __builtin_amdgcn_readfirstlane(v);
         return __shfl(v, (int)gbase, 64);
@@ -126,7 +159,7 @@ template <int NT>
 struct BlockGroup {
     static constexpr int kSize = NT;
     static constexpr int kWaves = NT / 64;
-    static constexpr int kScratchBytes = 64;
+    static constexpr int kScratchBytes = 256;
     uint32_t *scratch;
     uint32_t tid, lane, wave;
     __device__ explicit BlockGroup(void *s) : scratch((uint32_t *)s) {
@@ -202,6 +235,73 @@ struct BlockGroup {
         total = all;
         any_single = anys;
     }
+    template <int K>
+    __device__ void exchange_flags(bool single, const bool (&f)[K], bool &any_single, uint32_t &prefix,
+                                   uint32_t &total) const {
+        const uint64_t bs = __ballot(single);
+        uint32_t pre = 0, tot = 0;
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            const uint64_t m = __ballot(f[j]);
+            pre = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, pre));
+            tot += (uint32_t)__popcll(m);
+        }
+        __syncthreads();
+        if (lane == 0) {
+            scratch[wave] = tot;
+            scratch[kWaves + wave] = bs != 0;
+        }
+        __syncthreads();
+        uint32_t wpre = 0, all = 0;
+        bool anys = false;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+            const uint32_t x = scratch[w];
+            wpre += (w < (int)wave) ? x : 0u;
+            all += x;
+            anys |= scratch[kWaves + w] != 0;
+        }
+        prefix = wpre + pre;
+        total = __builtin_amdgcn_readfirstlane(all);
+        any_single = __builtin_amdgcn_readfirstlane(anys ? 1u : 0u) != 0u;
+    }
+    template <int K>
+    __device__ void place(bool single, const bool (&f)[K], uint32_t base, bool &any_single, uint32_t (&pos)[K],
+                          uint32_t &total) const {
+        const uint64_t bs = __ballot(single);
+        uint32_t pre[K], cnt[K];
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            const uint64_t m = __ballot(f[j]);
+            pre[j] = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+            cnt[j] = (uint32_t)__popcll(m);
+        }
+        __syncthreads();
+        if (lane == 0) {
+#pragma unroll
+            for (int j = 0; j < K; ++j) scratch[wave * K + j] = cnt[j];
+            scratch[kWaves * K + wave] = bs != 0;
+        }
+        __syncthreads();
+        bool anys = false;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) anys |= scratch[kWaves * K + w] != 0;
+        uint32_t b = base;
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            uint32_t before = 0, tot = 0;
+#pragma unroll
+            for (int w = 0; w < kWaves; ++w) {
+                const uint32_t c = scratch[w * K + j];
+                before += (w < (int)wave) ? c : 0u;
+                tot += c;
+            }
+            pos[j] = b + before + pre[j];
+            b += tot;
+        }
+        total = __builtin_amdgcn_readfirstlane(b - base);
+        any_single = __builtin_amdgcn_readfirstlane(anys ? 1u : 0u) != 0u;
+    }
     __device__ uint32_t bcast0(uint32_t v) const {
         __syncthreads();
         if (tid == 0) scratch[2 * kWaves] = v;
@@ -249,8 +349,10 @@ __device__ int64_t fetch_work(const BatchParams &P, const Grp &g, uint32_t &q, i
     return it++;
 }
 
-// Sets up the segments of a batch of `count` work items.
-__global__ void init_queues_kernel(uint32_t *queues, int64_t count) {
+// Sets up the segments of a batch of `count` work items (read from
+// count_dev when it is not null).
+__global__ void init_queues_kernel(uint32_t *queues, int64_t count, const uint32_t *count_dev) {
+    if (count_dev) count = (int64_t)*count_dev;
     const int k = threadIdx.x;
     if (k < kQueues) {
         const uint32_t s = (uint32_t)(count * k / kQueues), e = (uint32_t)(count * (k + 1) / kQueues);
@@ -295,6 +397,11 @@ struct GroupSolver {
     uint32_t done_local;
     uint64_t subsets_local;
 
+    static constexpr bool kNestedLoop = false;
+    // LDS: one region of lds_group_bytes per group, then the group scratch
+    __device__ GroupSolver(const BatchParams &p, uint8_t *smem, uint32_t gib, uint32_t groups, uint8_t *slot)
+        : GroupSolver(p, smem + (uint64_t)groups * p.lds_group_bytes, smem + (uint64_t)gib * p.lds_group_bytes,
+                      slot) {}
     __device__ GroupSolver(const BatchParams &p, void *scratch, uint8_t *gsm, uint8_t *slot)
         : P(p), g(scratch) {
         tab = (lds_u64 *)gsm;
@@ -354,6 +461,7 @@ struct GroupSolver {
         nx_ai = item < 0 ? -1 : (P.order ? (int64_t)P.order[item] : item);
     }
     __device__ bool has_next() const { return nx_ai >= 0; }
+    __device__ int begin() { return start() ? (int)kContinue : -1; }
     __device__ bool start() {
         const int64_t a = nx_ai;
         prefetch();
@@ -590,29 +698,35 @@ struct GroupSolver {
 
 // Small automata (n <= 16) without witness: the whole live frontier sits in an
 // LDS ring of 16-bit subsets (ring_cap entries; the largest two-level window
-// of C_16 is 3099 subsets), images come from four packed nibble tables
-// (a | b << 16), and every lane takes T frontier entries per level step.  A
-// frontier that outgrows the ring moves to a 65536-entry ring in the group's
-// HBM slot, which no n <= 16 frontier can outgrow.
-template <class Grp, int T>
+// of C_16 is 3099 subsets), images come from packed tables (a | b << 16) of
+// four nibble slices (BT = false, 256 bytes) or two byte slices (BT = true,
+// 2 KiB, half the lookups), and every lane takes up to T frontier entries per
+// level step.  A frontier that outgrows the ring is either listed for a
+// second launch with a larger ring (spill) or moves to a 65536-entry ring in
+// the group's HBM slot, which no n <= 16 frontier can outgrow.
+//
+// LDS (SmallLds): the bitmaps of all groups of the workgroup come first, each
+// aligned to its size, then one region per group (table, transitions
+// staging, singleton images, ring plus 64 dummy entries).  With aligned bases
+// a bitmap word or table entry address is one OR of the base and the scaled
+// index; lanes without a new entry write a dummy ring entry instead of
+// branching around the write.
+template <class Grp, int T, bool BT>
 struct SmallSolver {
     static constexpr uint32_t G = Grp::kSize;
-    static constexpr int kBits = (2 * T >= 8) ? 4 : (2 * T >= 4 ? 3 : 2);
     static constexpr uint32_t kGlobalRing = 65536;
-    static constexpr uint32_t kStage = 64;  // staging bytes for the transitions
+    // one persistent loop per automaton when a group is a whole wavefront or
+    // more (its control flow is uniform)
+    static constexpr bool kNestedLoop = G >= 64;
     const BatchParams &P;
     Grp g;
-    lds_u32 *tab;
-    lds_u8 *stage;
-    lds_u32 *bm;
-    lds_u16 *ring;
+    uint32_t tab_a, stage_a, pimg_a, ring_a, bm_a;  // LDS byte addresses
     glb_u16 *gring;
-    uint32_t rmask;
+    uint32_t rmask, rcap;
     bool in_lds;
     int64_t ai;
     int32_t n;
     uint32_t Q;
-    int32_t nsl;
     uint32_t hi, wr, rd, level;
     int64_t it, it_end;
     uint32_t wq;
@@ -624,14 +738,20 @@ struct SmallSolver {
     int32_t nx_n;
     uint32_t nx_b0, nx_b1;
 
-    __device__ SmallSolver(const BatchParams &p, void *scratch, uint8_t *gsm, uint8_t *slot)
-        : P(p), g(scratch) {
-        tab = (lds_u32 *)gsm;
-        stage = (lds_u8 *)(gsm + 64 * 4);
-        bm = (lds_u32 *)(gsm + 64 * 4 + kStage);
-        ring = (lds_u16 *)(gsm + 64 * 4 + kStage + 4u * P.lds_bitmap_words);
+    __device__ SmallSolver(const BatchParams &p, uint8_t *smem, uint32_t gib, uint32_t groups, uint8_t *slot)
+        : P(p), g(smem + SmallLds::block_bytes(groups, 4u * p.lds_bitmap_words, p.lds_group_bytes, 0)) {
+        const uint32_t base = uni<G>((uint32_t)(size_t)(lds_u8 *)smem);
+        const uint32_t bm_bytes = 4u * P.lds_bitmap_words;
+        // kept in VGPRs: an address is then one v_and_or of the scaled index,
+        // a mask in an SGPR and the base (one scalar operand per instruction)
+        bm_a = base + gib * bm_bytes;
+        tab_a = base + SmallLds::regions_base(groups, bm_bytes) + gib * P.lds_group_bytes;
+        stage_a = tab_a + SmallLds::tab_bytes(BT);
+        pimg_a = stage_a + 64;
+        ring_a = pimg_a + 64;
         gring = (glb_u16 *)(slot + P.garena_off);
-        rmask = P.lds_arena_cap - 1u;
+        rcap = P.lds_arena_cap;
+        rmask = rcap - 1u;
         it = 0;
         it_end = 0;
         wq = 0;
@@ -640,9 +760,11 @@ struct SmallSolver {
         subsets_local = 0;
         nx_ai = -1;
     }
-    static __host__ __device__ size_t lds_bytes(uint32_t bitmap_words, uint32_t ring_cap) {
-        return 64 * 4 + kStage + 4ull * bitmap_words + ((2ull * ring_cap + 15) & ~15ull);
-    }
+    static __device__ __forceinline__ lds_u32 *l32(uint32_t a) { return (lds_u32 *)(size_t)a; }
+    static __device__ __forceinline__ lds_u16 *l16(uint32_t a) { return (lds_u16 *)(size_t)a; }
+    static __device__ __forceinline__ lds_u8 *l8(uint32_t a) { return (lds_u8 *)(size_t)a; }
+    static __device__ __forceinline__ lds_u4 *l128(uint32_t a) { return (lds_u4 *)(size_t)a; }
+
     // LDS-only ordering inside one wavefront needs no memory fence: LDS
     // instructions of a wave complete in order
     __device__ __forceinline__ void lsync() const {
@@ -654,24 +776,17 @@ struct SmallSolver {
 
     template <bool LDS>
     __device__ __forceinline__ uint32_t rget(uint32_t p) const {
-        return LDS ? (uint32_t)ring[p & rmask] : (uint32_t)gring[p & (kGlobalRing - 1u)];
+        return LDS ? (uint32_t)*l16(ring_a + ((p & rmask) << 1)) : (uint32_t)gring[p & (kGlobalRing - 1u)];
     }
-    template <bool LDS>
-    __device__ __forceinline__ void rput(uint32_t p, uint32_t v) const {
-        if (LDS)
-            ring[p & rmask] = (uint16_t)v;
-        else
-            gring[p & (kGlobalRing - 1u)] = (uint16_t)v;
-    }
-    // all four slices are built (slices past n hold zeros), so the lookups
-    // are unconditional and issue back to back
+    // all slices are built (slices past n hold zeros), so the lookups are
+    // unconditional and issue back to back
     __device__ __forceinline__ uint32_t image(uint32_t s) const {
-        return tab[s & 15u] | tab[16u | ((s >> 4) & 15u)] | tab[32u | ((s >> 8) & 15u)] |
-               tab[48u | ((s >> 12) & 15u)];
-    }
-    __device__ __forceinline__ bool tas(uint32_t x) const {
-        const uint32_t bit = 1u << (x & 31u);
-        return (or_wg(bm + (x >> 5), bit) & bit) == 0u;
+        if constexpr (BT) {
+            return *l32(tab_a | ((s << 2) & 0x3FCu)) | *l32((tab_a | ((s >> 6) & 0x3FCu)) + 1024u);
+        } else {
+            return *l32(tab_a | ((s << 2) & 60u)) | *l32((tab_a | ((s >> 2) & 60u)) + 64u) |
+                   *l32((tab_a | ((s >> 6) & 60u)) + 128u) | *l32((tab_a | ((s >> 10) & 60u)) + 192u);
+        }
     }
 
     __device__ int64_t next_item() {
@@ -709,8 +824,12 @@ struct SmallSolver {
     }
     __device__ bool has_next() const { return nx_ai >= 0; }
 
-    // Starts the prefetched automaton; false when its input is invalid.
-    __device__ bool start() {
+    // Starts the prefetched automaton: -1 when its input is invalid (the
+    // result is written), kNoSync when n = 1 (Q is the only subset and
+    // already a singleton: not synchronizing, one subset expanded), else
+    // kContinue.
+    __device__ int begin() { return start(); }
+    __device__ int start() {
         ai = nx_ai;
         n = uni<G>(nx_n);
         const uint32_t r = g.rank();
@@ -718,72 +837,100 @@ struct SmallSolver {
         const bool bad_n = n < 1 || n > P.lds_n;
         const uint32_t two_n = bad_n ? 0u : 2u * (uint32_t)n;
         bool badd = (r < two_n && b0 >= (uint32_t)n) || (G < 32 && r + G < two_n && b1 >= (uint32_t)n);
-        if (r < two_n) stage[r] = (uint8_t)b0;
-        if (G < 32 && r + G < two_n) stage[r + G] = (uint8_t)b1;
+        if (r < two_n) *l8(stage_a + r) = (uint8_t)b0;
+        if (G < 32 && r + G < two_n) *l8(stage_a + r + G) = (uint8_t)b1;
         prefetch();
         in_lds = true;
         if (g.any(bad_n || badd)) {
             if (r == 0) write_invalid();
-            return false;
+            return -1;
         }
-        lsync();
-        nsl = (n + 3) >> 2;
         Q = uni<G>((1u << n) - 1u);
+        lsync();
+        // singleton images of the 16 states, zero past n
+        if (r < 16) {
+            uint32_t pi = 0;
+            if (r < (uint32_t)n) pi = (1u << *l8(stage_a + r)) | (1u << (16u + *l8(stage_a + (uint32_t)n + r)));
+            *l32(pimg_a + 4u * r) = pi;
+        }
         const uint32_t words = n >= 5 ? (1u << (n - 5)) : 1u;
         if (words >= 4) {
-            lds_u4 *b4 = (lds_u4 *)bm;
-            for (uint32_t w = r; w < (words >> 2); w += G) b4[w] = u32x4{0u, 0u, 0u, 0u};
+            for (uint32_t w = r; w < (words >> 2); w += G) *l128(bm_a + 16u * w) = u32x4{0u, 0u, 0u, 0u};
         } else {
-            for (uint32_t w = r; w < words; w += G) bm[w] = 0;
+            for (uint32_t w = r; w < words; w += G) *l32(bm_a + 4u * w) = 0u;
         }
-        for (int e = (int)r; e < 64; e += (int)G) {
-            const int k = e >> 4, v = e & 15;
-            uint32_t acc = 0;
-#pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const int q = 4 * k + t;
-                if (q < n && ((v >> t) & 1)) acc |= (1u << stage[q]) | (1u << (16 + stage[n + q]));
+        lsync();
+        if constexpr (BT) {
+            // 64 blocks of 8 entries: slice k = blk >> 5, entries v = 8 * (blk & 31) + i
+            for (uint32_t blk = r; blk < 64u; blk += G) {
+                const uint32_t k = blk >> 5, h5 = blk & 31u;
+                const u32x4 lo4 = *l128(pimg_a + 32u * k), hi4 = *l128(pimg_a + 32u * k + 16u);
+                const uint32_t h = ((h5 & 1u) ? lo4.w : 0u) | ((h5 & 2u) ? hi4.x : 0u) | ((h5 & 4u) ? hi4.y : 0u) |
+                                   ((h5 & 8u) ? hi4.z : 0u) | ((h5 & 16u) ? hi4.w : 0u);
+                const uint32_t p01 = lo4.x | lo4.y;
+                const uint32_t e = tab_a + 1024u * k + 32u * h5;
+                *l128(e) = u32x4{h, h | lo4.x, h | lo4.y, h | p01};
+                *l128(e + 16u) = u32x4{h | lo4.z, h | lo4.x | lo4.z, h | lo4.y | lo4.z, h | p01 | lo4.z};
             }
-            tab[e] = acc;
+        } else {
+            for (uint32_t e = r; e < 64u; e += G) {
+                const uint32_t v = e & 15u;
+                const u32x4 p4 = *l128(pimg_a + 16u * (e >> 4));
+                *l32(tab_a + 4u * e) = ((v & 1u) ? p4.x : 0u) | ((v & 2u) ? p4.y : 0u) | ((v & 4u) ? p4.z : 0u) |
+                                       ((v & 8u) ? p4.w : 0u);
+            }
         }
         lsync();
         if (r == 0) {
-            or_wg(bm + (Q >> 5), 1u << (Q & 31u));
-            ring[0] = (uint16_t)Q;
+            or_wg(l32(bm_a | ((Q >> 3) & 0x1FFCu)), 1u << (Q & 31u));
+            *l16(ring_a) = (uint16_t)Q;
         }
         lsync();
         hi = 1;
         wr = 1;
         rd = 0;
         level = 0;
-        return true;
+        return n == 1 ? (int)kNoSync : (int)kContinue;
     }
 
     __device__ void migrate() {
-        for (uint32_t p = rd + g.rank(); p < wr; p += G) gring[p & (kGlobalRing - 1u)] = ring[p & rmask];
+        for (uint32_t p = rd + g.rank(); p < wr; p += G) gring[p & (kGlobalRing - 1u)] = (uint16_t)rget<true>(p);
         g.sync();
         in_lds = false;
     }
 
     __device__ int step() {
-        if (in_lds && wr + 2u * G * T - rd > P.lds_arena_cap) migrate();
         // entries per lane follow the width left in the level (wide C_n levels
-        // take T per lane, narrow random-automaton levels one); one code path
-        // per ring location, so the LDS path carries no wait on outstanding
-        // HBM loads (the prefetched next automaton)
+        // take T per lane, narrow random-automaton levels one)
         const uint32_t left = hi - rd;
+        const uint32_t tt = (T >= 4 && left >= 4u * G) ? 4u : ((T >= 2 && left >= 2u * G) ? 2u : 1u);
+        // the ring holds the unread window plus two images per entry taken
+        if (in_lds && wr + 2u * G * tt - rd > rcap) {
+            if (P.spill) return spill();
+            migrate();
+        }
+        // one code path per ring location, so the LDS path carries no wait on
+        // outstanding HBM loads (the prefetched next automaton)
         if constexpr (T >= 4) {
-            if (left >= 4u * G) return in_lds ? step_in<true, 4>() : step_in<false, 4>();
+            if (tt == 4u) return in_lds ? step_in<true, 4>() : step_in<false, 4>();
         }
         if constexpr (T >= 2) {
-            if (left >= 2u * G) return in_lds ? step_in<true, 2>() : step_in<false, 2>();
+            if (tt == 2u) return in_lds ? step_in<true, 2>() : step_in<false, 2>();
         }
         return in_lds ? step_in<true, 1>() : step_in<false, 1>();
     }
 
+    // Hands the automaton to the second launch (larger LDS ring).
+    __device__ int spill() {
+        if (g.rank() == 0) {
+            const uint32_t k = atomicAdd(P.spill_count, 1u);
+            P.spill_list[k] = (uint32_t)ai;
+        }
+        return kSpilled;
+    }
+
     template <bool LDS, int TT>
     __device__ int step_in() {
-        constexpr int kBitsTT = (2 * TT >= 8) ? 4 : (2 * TT >= 4 ? 3 : 2);
         const uint32_t r = g.rank();
         uint32_t s[TT];
         bool v[TT];
@@ -791,48 +938,52 @@ struct SmallSolver {
         for (int t = 0; t < TT; ++t) {
             const uint32_t p = rd + t * G + r;
             v[t] = p < hi;
-            // lanes past the level read a harmless slot and are masked below
+            // a lane past the level repeats entry rd: its singleton test is
+            // that of the lane owning rd, and it skips the test-and-sets
             s[t] = rget<LDS>(v[t] ? p : rd);
-            s[t] = v[t] ? s[t] : 0u;
         }
         uint32_t img[TT];
 #pragma unroll
         for (int t = 0; t < TT; ++t) img[t] = image(s[t]);
         bool single = false;
-        uint32_t c = 0, newmask = 0;
+        bool nw[2 * TT];
 #pragma unroll
         for (int t = 0; t < TT; ++t) {
-            const uint32_t a = img[t] & 0xFFFFu, b = img[t] >> 16;
-            const bool sa = v[t] && is_single(a) && a != Q;
-            const bool sb = v[t] && is_single(b) && b != Q;
-            single |= sa | sb;
-            // both test-and-sets of a lane issue back to back; lanes past the
-            // level skip them (a singleton image tests the empty subset's bit,
-            // which no search ever sets)
-            const uint32_t xa = sa ? 0u : a, xb = sb ? 0u : b;
-            const uint32_t ba = 1u << (xa & 31u), bb = 1u << (xb & 31u);
+            // a = low half, b = high half, both nonempty; y holds a & (a - 1)
+            // and b & (b - 1) in its halves.  Q is never a singleton here
+            // (n = 1 ends at start), so a singleton image is always new; it is
+            // still marked and appended, which is harmless because the search
+            // stops at this level.
+            const uint32_t x = img[t];
+            const uint32_t y = x & (x - 0x00010001u);
+            single |= ((uint16_t)y == 0) | (y < 0x10000u);
+            const uint32_t ba = 1u << (x & 31u), bb = 1u << ((x >> 16) & 31u);
             uint32_t oa = ~0u, ob = ~0u;
             if (v[t]) {
-                oa = or_wg(bm + (xa >> 5), ba);
-                ob = or_wg(bm + (xb >> 5), bb);
+                // issued back to back; the second sees the first, so a == b
+                // is counted once
+                // word of a: bits 5..15 of x; word of b: bits 21..31
+                oa = or_wg(l32(bm_a | ((x >> 3) & 0x1FFCu)), ba);
+                ob = or_wg(l32(bm_a | ((x >> 19) & 0x1FFCu)), bb);
             }
-            const bool na = xa != 0u && (oa & ba) == 0u;
-            const bool nb = xb != 0u && (ob & bb) == 0u && xb != xa;
-            newmask |= (na ? 1u : 0u) << (2 * t);
-            newmask |= (nb ? 1u : 0u) << (2 * t + 1);
-            c += (na ? 1u : 0u) + (nb ? 1u : 0u);
+            nw[2 * t] = (oa & ba) == 0u;
+            nw[2 * t + 1] = (ob & bb) == 0u;
         }
         bool any_single;
-        uint32_t prefix, total;
-        g.template exchange_count<kBitsTT>(single, c, any_single, prefix, total);
-        uint32_t k = wr + prefix;
+        uint32_t pos[2 * TT], total;
+        g.template place<2 * TT>(single, nw, wr, any_single, pos, total);
 #pragma unroll
-        for (int t = 0; t < TT; ++t) {
-            if ((newmask >> (2 * t)) & 1u) rput<LDS>(k++, img[t] & 0xFFFFu);
-            if ((newmask >> (2 * t + 1)) & 1u) rput<LDS>(k++, img[t] >> 16);
+        for (int j = 0; j < 2 * TT; ++j) {
+            const uint32_t val = (j & 1) ? (img[j >> 1] >> 16) : (img[j >> 1] & 0xFFFFu);
+            if constexpr (LDS) {
+                const uint32_t q = nw[j] ? (pos[j] & rmask) : (rcap + (r & 63u));
+                *l16(ring_a + (q << 1)) = (uint16_t)val;
+            } else {
+                if (nw[j]) gring[pos[j] & (kGlobalRing - 1u)] = (uint16_t)val;
+            }
         }
         wr = uni<G>(wr + total);
-        if (uni<G>(any_single ? 1u : 0u)) return kFound;
+        if (any_single) return kFound;
         rd = uni<G>(rd + G * TT);
         if (rd >= hi) {
             if (G > 64 || !LDS)
@@ -848,6 +999,10 @@ struct SmallSolver {
     }
 
     __device__ void finish(int code) {
+        if (code == kSpilled) {
+            lsync();
+            return;
+        }
         const bool found = code == kFound;
         const int32_t L = found ? (int32_t)level + 1 : -1;
         const int64_t E = hi;
@@ -880,32 +1035,45 @@ __global__ __launch_bounds__(256) void batch_kernel(BatchParams P) {
     constexpr uint32_t G = Grp::kSize;
     const uint32_t groups_per_block = blockDim.x / G;
     const uint32_t gib = threadIdx.x / G;
-    const uint32_t per_group = (uint32_t)P.lds_group_bytes;
-    uint8_t *gsm = smem + (uint64_t)gib * per_group;
-    void *scratch = smem + (uint64_t)groups_per_block * per_group;
     const uint64_t slot = (uint64_t)blockIdx.x * groups_per_block + gib;
-    Solver S(P, scratch, gsm, P.slot_base + slot * P.slot_bytes);
+    Solver S(P, smem, gib, groups_per_block, P.slot_base + slot * P.slot_bytes);
     S.wq = (uint32_t)(slot & (kQueues - 1));
-
-    bool active = true, need_new = true;
     S.prefetch();
-    for (;;) {
-        if (active && need_new) {
-            if (!S.has_next())
-                active = false;
-            else
-                need_new = !S.start();
+    if constexpr (Solver::kNestedLoop) {
+        // uniform control flow: one automaton at a time, one tight step loop
+        while (S.has_next()) {
+            int code = S.begin();
+            if (code < 0) continue;
+            while (code == kContinue) code = S.step();
+            S.finish(code);
         }
-        if (G < 64) {
-            if (__ballot(active) == 0) break;
-        } else if (!active) {
-            break;
-        }
-        if (active && !need_new) {
-            const int code = S.step();
-            if (code != kContinue) {
-                S.finish(code);
-                need_new = true;
+    } else {
+        // several groups per wavefront: each group starts its next automaton
+        // while the others keep stepping
+        bool active = true, need_new = true;
+        for (;;) {
+            if (active && need_new) {
+                if (!S.has_next()) {
+                    active = false;
+                } else {
+                    const int c = S.begin();
+                    if (c == kContinue)
+                        need_new = false;
+                    else if (c > 0)
+                        S.finish(c);
+                }
+            }
+            if (G < 64) {
+                if (__ballot(active) == 0) break;
+            } else if (!active) {
+                break;
+            }
+            if (active && !need_new) {
+                const int code = S.step();
+                if (code != kContinue) {
+                    S.finish(code);
+                    need_new = true;
+                }
             }
         }
     }
@@ -937,25 +1105,25 @@ static hipError_t occ_t(int block, size_t lds, int *blocks) {
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, batch_kernel<Solver, Grp>, block, lds);
 }
 
-// variant: 0 = general (witness, HBM bitmaps), 1..3 = small path with T = 1, 2, 4
+// variant: 0 = general (witness, HBM bitmaps); 1..3 = small path with T = 1, 2, 4
+// and nibble tables; 4..6 = small path with T = 1, 2, 4 and byte tables
+#define SSW_SMALL_CASES(FN, V, TT, BT, ...)                                                                   \
+    case V * 1000 + 16: return FN<SmallSolver<WaveGroup<16>, TT, BT>, WaveGroup<16>>(__VA_ARGS__);            \
+    case V * 1000 + 32: return FN<SmallSolver<WaveGroup<32>, TT, BT>, WaveGroup<32>>(__VA_ARGS__);            \
+    case V * 1000 + 64: return FN<SmallSolver<WaveGroup<64>, TT, BT>, WaveGroup<64>>(__VA_ARGS__);            \
+    case V * 1000 + 256: return FN<SmallSolver<BlockGroup<256>, TT, BT>, BlockGroup<256>>(__VA_ARGS__);
 #define SSW_DISPATCH(FN, ...)                                                              \
     switch (variant * 1000 + group_lanes) {                                                \
         case 16: return FN<GroupSolver<WaveGroup<16>>, WaveGroup<16>>(__VA_ARGS__);         \
         case 32: return FN<GroupSolver<WaveGroup<32>>, WaveGroup<32>>(__VA_ARGS__);         \
         case 64: return FN<GroupSolver<WaveGroup<64>>, WaveGroup<64>>(__VA_ARGS__);         \
         case 256: return FN<GroupSolver<BlockGroup<256>>, BlockGroup<256>>(__VA_ARGS__);    \
-        case 1016: return FN<SmallSolver<WaveGroup<16>, 1>, WaveGroup<16>>(__VA_ARGS__);    \
-        case 1032: return FN<SmallSolver<WaveGroup<32>, 1>, WaveGroup<32>>(__VA_ARGS__);    \
-        case 1064: return FN<SmallSolver<WaveGroup<64>, 1>, WaveGroup<64>>(__VA_ARGS__);    \
-        case 1256: return FN<SmallSolver<BlockGroup<256>, 1>, BlockGroup<256>>(__VA_ARGS__); \
-        case 2016: return FN<SmallSolver<WaveGroup<16>, 2>, WaveGroup<16>>(__VA_ARGS__);    \
-        case 2032: return FN<SmallSolver<WaveGroup<32>, 2>, WaveGroup<32>>(__VA_ARGS__);    \
-        case 2064: return FN<SmallSolver<WaveGroup<64>, 2>, WaveGroup<64>>(__VA_ARGS__);    \
-        case 2256: return FN<SmallSolver<BlockGroup<256>, 2>, BlockGroup<256>>(__VA_ARGS__); \
-        case 3016: return FN<SmallSolver<WaveGroup<16>, 4>, WaveGroup<16>>(__VA_ARGS__);    \
-        case 3032: return FN<SmallSolver<WaveGroup<32>, 4>, WaveGroup<32>>(__VA_ARGS__);    \
-        case 3064: return FN<SmallSolver<WaveGroup<64>, 4>, WaveGroup<64>>(__VA_ARGS__);    \
-        case 3256: return FN<SmallSolver<BlockGroup<256>, 4>, BlockGroup<256>>(__VA_ARGS__); \
+        SSW_SMALL_CASES(FN, 1, 1, false, __VA_ARGS__)                                      \
+        SSW_SMALL_CASES(FN, 2, 2, false, __VA_ARGS__)                                      \
+        SSW_SMALL_CASES(FN, 3, 4, false, __VA_ARGS__)                                      \
+        SSW_SMALL_CASES(FN, 4, 1, true, __VA_ARGS__)                                       \
+        SSW_SMALL_CASES(FN, 5, 2, true, __VA_ARGS__)                                       \
+        SSW_SMALL_CASES(FN, 6, 4, true, __VA_ARGS__)                                       \
         default: return hipErrorInvalidValue;                                              \
     }
 
@@ -972,13 +1140,22 @@ size_t batch_scratch_bytes(int group_lanes) {
     return group_lanes == 256 ? (size_t)BlockGroup<256>::kScratchBytes : 0;
 }
 
+// LDS bytes of one workgroup of `groups` groups (with the group scratch), and
+// the per-group figure the kernel reads from BatchParams.lds_group_bytes.
 size_t batch_group_lds(int variant, uint32_t bitmap_words, uint32_t arena_cap) {
     return variant == 0 ? (size_t)(kTableBytes + 4ull * bitmap_words + 4ull * arena_cap)
-                        : SmallSolver<WaveGroup<64>, 1>::lds_bytes(bitmap_words, arena_cap);
+                        : (size_t)SmallLds::region_bytes(variant >= 4, arena_cap);
 }
 
-hipError_t init_queues(uint32_t *queues, int64_t count, hipStream_t st) {
-    hipLaunchKernelGGL(init_queues_kernel, dim3(1), dim3(kQueues), 0, st, queues, count);
+size_t batch_block_lds(int variant, int group_lanes, int groups, uint32_t bitmap_words, uint32_t arena_cap) {
+    const size_t scratch = batch_scratch_bytes(group_lanes);
+    if (variant == 0) return batch_group_lds(0, bitmap_words, arena_cap) * groups + scratch;
+    return SmallLds::block_bytes((uint32_t)groups, 4u * bitmap_words, SmallLds::region_bytes(variant >= 4, arena_cap),
+                                 (uint32_t)scratch);
+}
+
+hipError_t init_queues(uint32_t *queues, int64_t count, hipStream_t st, const uint32_t *count_dev) {
+    hipLaunchKernelGGL(init_queues_kernel, dim3(1), dim3(kQueues), 0, st, queues, count, count_dev);
     return hipGetLastError();
 }
 

@@ -56,7 +56,13 @@ struct ClassSpec {
     int plain_n = 0;
     hipStream_t stream = nullptr;     // null: the context's stream
     uint32_t *queues = nullptr;       // this class's device work queue
+    uint32_t *spill_count = nullptr;  // ring-spill counter and list (null: no spill pass)
+    uint32_t *spill_list = nullptr;
 };
+
+// launch passes of one class
+enum PassKind { kPassAll = 0, kPassSpillFrom = 1, kPassSpillTo = 2 };
+constexpr size_t kSpillCtrBytes = 256;
 
 struct SlotKey {
     int G, block, lds_n, nmax, witness, arena_lds, plain;
@@ -146,7 +152,7 @@ struct ssw_context {
     std::mutex monitor_mu;   // progress reads
     std::string err;
 
-    ssw::DevBuf ctrs, queues, deferred, order;
+    ssw::DevBuf ctrs, queues, deferred, order, spill;
     ssw::DevBuf d_delta, d_nper, d_off, d_len, d_sync, d_exp, d_wit;
     std::map<ssw::SlotKey, ssw::DevBuf> slot_cache;
     std::map<ssw::SlotKey, uint64_t> slot_gen;   // call that last used each entry
@@ -178,6 +184,7 @@ struct ssw_context {
     int ensure(ssw::DevBuf &b, size_t bytes, const char *what);
     int get_slots(const ssw::SlotKey &key, uint64_t bytes, ssw::DevBuf **out, hipStream_t st);
     int launch_class(const ssw::DeviceBatch &db, const ssw::ClassSpec &cls, bool plain);
+    int launch_pass(const ssw::DeviceBatch &db, const ssw::ClassSpec &cls, const ssw_tuning &t, bool plain, int pass);
     int solve_device(const ssw::DeviceBatch &db, const uint8_t *host_n, bool plain, int64_t *violation_out);
     int solve_host_packed(const ssw_batch *b, int mode, int64_t *violation_out);
     int solve_entry(const ssw_batch *b, int32_t memory, int mode, int64_t *violation_out);

@@ -20,7 +20,33 @@ constexpr int kQueueWords = 32;
 constexpr size_t kQueueBytes = kQueues * kQueueWords * 4 + 64;
 
 // Result codes of one BFS level step.
-enum StepCode : int { kContinue = 0, kFound = 1, kNoSync = 2, kOverflow = 3 };
+enum StepCode : int { kContinue = 0, kFound = 1, kNoSync = 2, kOverflow = 3, kSpilled = 4 };
+
+// LDS layout of the n <= 16 path, per workgroup: the visited bitmaps of all
+// groups first, each aligned to its size (a power of two, at least 256
+// bytes), then one region per group -- image table (four nibble slices,
+// 256 bytes, or two byte slices, 2 KiB), transitions staging (64 bytes),
+// singleton images of the states (64 bytes), and the frontier ring of
+// ring_cap u16 entries plus 64 dummy entries -- then the group scratch.
+struct SmallLds {
+    static __host__ __device__ inline uint32_t bitmap_bytes(int n) {
+        const uint32_t b = n >= 3 ? (1u << (n - 3)) : 1u;
+        return b < 256u ? 256u : b;
+    }
+    static __host__ __device__ inline uint32_t tab_bytes(bool byte_tab) { return byte_tab ? 2048u : 256u; }
+    static __host__ __device__ inline uint32_t region_bytes(bool byte_tab, uint32_t ring_cap) {
+        const uint32_t b = tab_bytes(byte_tab) + 64u + 64u + 2u * (ring_cap + 64u);
+        const uint32_t a = byte_tab ? 1024u : 256u;
+        return (b + a - 1u) / a * a;
+    }
+    static __host__ __device__ inline uint32_t regions_base(uint32_t groups, uint32_t bm_bytes) {
+        return (groups * bm_bytes + 1023u) & ~1023u;
+    }
+    static __host__ __device__ inline uint32_t block_bytes(uint32_t groups, uint32_t bm_bytes, uint32_t region,
+                                                           uint32_t scratch) {
+        return regions_base(groups, bm_bytes) + groups * region + scratch;
+    }
+};
 
 // Arguments of the batched persistent kernel (and of the plain path, which is
 // the same kernel run on one automaton per launch by one 256-thread group).
@@ -60,6 +86,13 @@ struct BatchParams {
     uint32_t chunk;                    // automata per work-queue fetch
     uint32_t single_item;              // plain path: solve only first_item
     int64_t first_item;
+    // Ring spill (n <= 16 path): with spill = 1 an automaton whose frontier
+    // outgrows the LDS ring is not moved to HBM; its index is appended to
+    // spill_list (spill_count entries) and a second launch with a larger ring
+    // solves it.  The second launch reads its item count from spill_count.
+    uint32_t spill;
+    uint32_t *spill_count;
+    uint32_t *spill_list;
 };
 
 // Device-wide path: every instance keeps its visited bitmap (2^n bits) and a

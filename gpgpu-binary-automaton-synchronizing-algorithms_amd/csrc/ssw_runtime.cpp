@@ -27,8 +27,9 @@ hipError_t launch_batch(int variant, int group_lanes, const BatchParams &P, int 
 hipError_t batch_occupancy(int variant, int group_lanes, int block, size_t lds, int *blocks);
 size_t batch_scratch_bytes(int group_lanes);
 size_t batch_group_lds(int variant, uint32_t bitmap_words, uint32_t arena_cap);
+size_t batch_block_lds(int variant, int group_lanes, int groups, uint32_t bitmap_words, uint32_t arena_cap);
 hipError_t fill_u32(uint32_t *p, uint64_t words, uint32_t value, hipStream_t st);
-hipError_t init_queues(uint32_t *queues, int64_t count, hipStream_t st);
+hipError_t init_queues(uint32_t *queues, int64_t count, hipStream_t st, const uint32_t *count_dev);
 }  // namespace ssw
 
 using namespace ssw;
@@ -115,8 +116,8 @@ int ssw_context::open(int dev, uint64_t mem_budget) {
 void ssw_context::close() {
     stop_progress_thread();
     if (device >= 0) hipSetDevice(device);
-    for (DevBuf *b : {&ctrs, &queues, &deferred, &order, &d_delta, &d_nper, &d_off, &d_len, &d_sync, &d_exp,
-                      &d_wit})
+    for (DevBuf *b : {&ctrs, &queues, &deferred, &order, &spill, &d_delta, &d_nper, &d_off, &d_len, &d_sync,
+                      &d_exp, &d_wit})
         if (b->p) hipFree(b->p);
     for (auto &kv : slot_cache)
         if (kv.second.p) hipFree(kv.second.p);
@@ -197,8 +198,25 @@ static inline uint32_t next_pow2(uint64_t x) {
     return (uint32_t)p;
 }
 
+// One launch class.  With ring spill on (n <= 16 path, spill_arena_lds > 0)
+// it is two launches on the class's stream: the tuned configuration, in which
+// automata whose frontier outgrows the small LDS ring are listed instead of
+// moved to HBM, then a launch with the larger ring over that list.
 int ssw_context::launch_class(const DeviceBatch &db, const ClassSpec &cls, bool plain) {
-    const ssw_tuning t = plain ? ssw_tuning{256, 256, 1, 8192, 1} : tune[cls.nmax];
+    const ssw_tuning t = plain ? ssw_tuning{256, 256, 1, 8192, 1, 0, 0} : tune[cls.nmax];
+    const bool small = cls.lds && db.witness == nullptr && cls.nmax <= kMaxLdsN;
+    if (plain || !small || t.spill_arena_lds <= 0 || !cls.spill_count) return launch_pass(db, cls, t, plain, kPassAll);
+    int rc = launch_pass(db, cls, t, false, kPassSpillFrom);
+    if (rc) return rc;
+    ssw_tuning t2 = t;
+    t2.block_threads = 0;
+    t2.chunk = 1;
+    t2.arena_lds = t.spill_arena_lds;
+    t2.lane_entries = t.spill_lane_entries > 0 ? t.spill_lane_entries : t.lane_entries;
+    return launch_pass(db, cls, t2, false, kPassSpillTo);
+}
+
+int ssw_context::launch_pass(const DeviceBatch &db, const ClassSpec &cls, const ssw_tuning &t, bool plain, int pass) {
     const int G = t.group_lanes;
     int block = G == 256 ? 256 : std::max(64, std::min(256, t.block_threads ? t.block_threads : 256));
     const bool use_lds = cls.lds;
@@ -207,17 +225,23 @@ int ssw_context::launch_class(const DeviceBatch &db, const ClassSpec &cls, bool 
     // the small path: bitmap in LDS, no witness, 16-bit ring frontier
     const bool small = use_lds && !want_wit && nmax <= kMaxLdsN;
     const int T = t.lane_entries >= 4 ? 4 : (t.lane_entries >= 2 ? 2 : 1);
-    const int variant = small ? (T == 1 ? 1 : (T == 2 ? 2 : 3)) : 0;
+    // the spill-to pass takes the wide searches: byte-sliced image tables
+    // (two lookups per subset instead of four)
+    const bool byte_tab = pass == kPassSpillTo;
+    const int variant = small ? (T == 1 ? 1 : (T == 2 ? 2 : 3)) + (byte_tab ? 3 : 0) : 0;
 
     BatchParams P;
     std::memset(&P, 0, sizeof P);
-    P.count = plain ? db.count : (int64_t)cls.items;
+    P.count = plain ? db.count : (int64_t)cls.items;  // an upper bound in the spill-to pass
     P.uniform_n = db.uniform_n;
     P.bound = db.bound;
     P.n_per = db.n_per;
     P.offset = db.offset;
     P.delta = db.delta;
-    P.order = cls.order;
+    P.order = pass == kPassSpillTo ? cls.spill_list : cls.order;
+    P.spill = pass == kPassSpillFrom ? 1u : 0u;
+    P.spill_count = cls.spill_count;
+    P.spill_list = cls.spill_list;
     P.length = db.length;
     P.sync = db.sync;
     P.expanded = db.expanded;
@@ -233,13 +257,15 @@ int ssw_context::launch_class(const DeviceBatch &db, const ClassSpec &cls, bool 
     P.deferred_list = (uint32_t *)deferred.p;
     P.chunk = (uint32_t)std::max(1, t.chunk);
     P.lds_n = use_lds ? nmax : 0;
-    const uint32_t bm_words = use_lds ? (uint32_t)align_up(nmax >= 5 ? (1u << (nmax - 5)) : 1u, 4) : 4u;
+    const uint32_t bm_words = small ? SmallLds::bitmap_bytes(nmax) / 4u
+                                    : (use_lds ? (uint32_t)align_up(nmax >= 5 ? (1u << (nmax - 5)) : 1u, 4) : 4u);
     P.lds_bitmap_words = bm_words;
     const uint64_t all_subsets = 1ull << nmax;
     uint32_t arena_lds;
     if (small) {
-        // a ring holding every subset of Q never has to leave LDS
-        arena_lds = next_pow2((uint64_t)std::max(t.arena_lds, 4 * G * T));
+        // a ring holding every subset of Q never has to leave LDS; a spilling
+        // pass only needs room for one narrow step
+        arena_lds = next_pow2((uint64_t)std::max(t.arena_lds, pass == kPassSpillFrom ? 4 * G : 4 * G * T));
         arena_lds = std::min(arena_lds, next_pow2(all_subsets + 2ull * G * T));
     } else {
         arena_lds = (uint32_t)align_up((uint64_t)std::max(4, t.arena_lds), 4);
@@ -255,7 +281,7 @@ int ssw_context::launch_class(const DeviceBatch &db, const ClassSpec &cls, bool 
         int best_groups = 0;
         for (int cand = 64; cand <= 256; cand += 64) {
             if (cand % G) continue;
-            const size_t l = per_group * (cand / G) + batch_scratch_bytes(G);
+            const size_t l = batch_block_lds(variant, G, cand / G, bm_words, arena_lds);
             if (l > lds_per_block) continue;
             int occ = 0;
             HIP_TRY(batch_occupancy(variant, G, cand, l, &occ), "occupancy");
@@ -267,10 +293,11 @@ int ssw_context::launch_class(const DeviceBatch &db, const ClassSpec &cls, bool 
         }
     } else {
         // fewer groups per workgroup when the tuned block does not fit the LDS limit
-        while (G < 256 && block > 64 && per_group * (block / G) > lds_per_block) block -= 64;
+        while (G < 256 && block > 64 && batch_block_lds(variant, G, block / G, bm_words, arena_lds) > lds_per_block)
+            block -= 64;
     }
     const int groups_per_block = block / G;
-    const size_t lds = per_group * groups_per_block + batch_scratch_bytes(G);
+    const size_t lds = batch_block_lds(variant, G, groups_per_block, bm_words, arena_lds);
     if (lds > lds_per_block)
         return fail(SSW_ERR_ARG, "tuning for n=%d needs %zu bytes of LDS per block (limit %zu)", nmax, lds,
                     lds_per_block);
@@ -320,7 +347,7 @@ int ssw_context::launch_class(const DeviceBatch &db, const ClassSpec &cls, bool 
         while ((uint64_t)grid * groups_per_block > max_slots && grid > 1) grid = grid / 2;
     }
     const uint64_t slots = (uint64_t)grid * groups_per_block;
-    SlotKey key{G, block, P.lds_n, nmax, (int)want_wit, (int)arena_lds, (plain ? 1 : 0) + 2 * variant};
+    SlotKey key{G, block, P.lds_n, nmax, (int)want_wit, (int)arena_lds, (plain ? 1 : 0) + 2 * variant + 8 * pass};
     DevBuf *sb = nullptr;
     int rc = get_slots(key, slots * P.slot_bytes, &sb, st);
     if (rc) return rc;
@@ -331,7 +358,7 @@ int ssw_context::launch_class(const DeviceBatch &db, const ClassSpec &cls, bool 
     last_variant = variant;
 
     if (!plain) {
-        HIP_TRY(init_queues(P.queues, P.count, st), "init queues");
+        HIP_TRY(init_queues(P.queues, P.count, st, pass == kPassSpillTo ? cls.spill_count : nullptr), "init queues");
         HIP_TRY(launch_batch(variant, G, P, grid, block, lds, st), "batch kernel launch");
         ++launches;
     } else {
@@ -430,6 +457,12 @@ int ssw_context::solve_device(const DeviceBatch &db, const uint8_t *host_n, bool
         }
         rc = ensure(queues, kQueueBytes * classes.size(), "work queues");
         if (rc) return rc;
+        // ring-spill lists: a counter per class (own 256-byte line), then one
+        // list slot per automaton at the automaton's position in class order
+        const size_t spill_ctr_bytes = kSpillCtrBytes * classes.size();
+        rc = ensure(spill, spill_ctr_bytes + 4ull * db.count, "spill lists");
+        if (rc) return rc;
+        HIP_TRY(fill_u32((uint32_t *)spill.p, spill_ctr_bytes / 4, 0, stream), "fill");
         const int nstreams = std::min<int>((int)classes.size(), kClassStreams);
         HIP_TRY(hipEventRecord(ev_fork, stream), "hipEventRecord");
         for (int k = 0; k < nstreams; ++k) HIP_TRY(hipStreamWaitEvent(class_streams[k], ev_fork, 0), "wait fork");
@@ -445,6 +478,8 @@ int ssw_context::solve_device(const DeviceBatch &db, const uint8_t *host_n, bool
             cls.plain_n = 0;
             cls.stream = nstreams > 1 ? class_streams[k % nstreams] : stream;
             cls.queues = (uint32_t *)((uint8_t *)queues.p + kQueueBytes * k);
+            cls.spill_count = (uint32_t *)((uint8_t *)spill.p + kSpillCtrBytes * k);
+            cls.spill_list = (uint32_t *)((uint8_t *)spill.p + spill_ctr_bytes) + class_start[k];
             rc = launch_class(db, cls, false);
             if (rc) return rc;
             ++k;
@@ -781,6 +816,11 @@ int ssw_set_tuning(ssw_context *ctx, int32_t n, const ssw_tuning *t) {
     if (t->chunk < 1 || t->arena_lds < 4) return ctx->fail(SSW_ERR_ARG, "chunk/arena_lds");
     if (t->lane_entries != 1 && t->lane_entries != 2 && t->lane_entries != 4)
         return ctx->fail(SSW_ERR_ARG, "lane_entries %d", t->lane_entries);
+    if (t->spill_arena_lds < 0 || t->spill_arena_lds > 65536)
+        return ctx->fail(SSW_ERR_ARG, "spill_arena_lds %d", t->spill_arena_lds);
+    if (t->spill_lane_entries != 0 && t->spill_lane_entries != 1 && t->spill_lane_entries != 2 &&
+        t->spill_lane_entries != 4)
+        return ctx->fail(SSW_ERR_ARG, "spill_lane_entries %d", t->spill_lane_entries);
     ctx->tune[n] = *t;
     return SSW_OK;
 }

@@ -5,7 +5,8 @@
 // with the shortest kernel time (HIP events on the context's stream, best of
 // two runs).  It searches one parameter at a time, starting from the current
 // table: threads per automaton, workgroup size, work-queue chunk, frontier
-// tile in LDS, frontier entries per lane.
+// tile in LDS, frontier entries per lane, and (n <= 16) the ring of the
+// second launch that takes the automata outgrowing the first one's ring.
 #include "ssw_host.h"
 
 #include <algorithm>
@@ -40,7 +41,7 @@ bool valid_config(const ssw_tuning &t) {
     if (G != 16 && G != 32 && G != 64 && G != 256) return false;
     if (t.block_threads && (t.block_threads % 64 || t.block_threads > 256 || t.block_threads % G)) return false;
     if (G == 256 && t.block_threads && t.block_threads != 256) return false;
-    return t.chunk >= 1 && t.arena_lds >= 4;
+    return t.chunk >= 1 && t.arena_lds >= 4 && t.spill_arena_lds >= 0 && t.spill_lane_entries >= 0;
 }
 
 // Tries the values of one parameter around `best`, keeps the fastest.
@@ -82,6 +83,24 @@ int tune_one(ssw_context &ctx, int n, const DeviceBatch &db, Timed *out) {
     if ((rc = sweep(ctx, n, db, best, tiles, [](ssw_tuning &t, int v) { t.arena_lds = v; }))) return rc;
     if (small && (rc = sweep(ctx, n, db, best, {1, 2, 4}, [](ssw_tuning &t, int v) { t.lane_entries = v; })))
         return rc;
+    if (small) {
+        // ring spill: automata that outgrow a small first-launch ring go to a
+        // second launch with a large ring; then the first ring can shrink
+        if ((rc = sweep(ctx, n, db, best, {0, 1024, 2048, 4096, 8192}, [](ssw_tuning &t, int v) {
+                 t.spill_arena_lds = v;
+                 if (v && !t.spill_lane_entries) t.spill_lane_entries = 4;
+             })))
+            return rc;
+        if (best.t.spill_arena_lds > 0) {
+            if ((rc = sweep(ctx, n, db, best, {128, 256, 512, 1024},
+                            [](ssw_tuning &t, int v) { t.arena_lds = v; })))
+                return rc;
+            if ((rc = sweep(ctx, n, db, best, {1, 2, 4}, [](ssw_tuning &t, int v) { t.lane_entries = v; })))
+                return rc;
+            if ((rc = sweep(ctx, n, db, best, {1, 2, 4}, [](ssw_tuning &t, int v) { t.spill_lane_entries = v; })))
+                return rc;
+        }
+    }
     *out = best;
     return SSW_OK;
 }
@@ -95,9 +114,9 @@ void write_json(FILE *f, ssw_context &ctx, int n_lo, int n_hi, int64_t samples, 
         const ssw_tuning &t = ctx.tune[n];
         fprintf(f,
                 "    \"%d\": {\"group_lanes\": %d, \"block_threads\": %d, \"chunk\": %d, \"arena_lds\": %d, "
-                "\"lane_entries\": %d, \"ms\": %.4f}%s\n",
-                n, t.group_lanes, t.block_threads, t.chunk, t.arena_lds, t.lane_entries, ms[n - n_lo],
-                n < n_hi ? "," : "");
+                "\"lane_entries\": %d, \"spill_arena_lds\": %d, \"spill_lane_entries\": %d, \"ms\": %.4f}%s\n",
+                n, t.group_lanes, t.block_threads, t.chunk, t.arena_lds, t.lane_entries, t.spill_arena_lds,
+                t.spill_lane_entries, ms[n - n_lo], n < n_hi ? "," : "");
     }
     fprintf(f, "  }\n}\n");
 }
@@ -207,6 +226,8 @@ int ssw_load_tuning(ssw_context *ctx, const char *path) {
         read_int(obj, "chunk", &t.chunk);
         read_int(obj, "arena_lds", &t.arena_lds);
         read_int(obj, "lane_entries", &t.lane_entries);
+        read_int(obj, "spill_arena_lds", &t.spill_arena_lds);
+        read_int(obj, "spill_lane_entries", &t.spill_lane_entries);
         if (n >= 1 && n <= SSW_MAX_N && valid_config(t)) {
             ctx->tune[n] = t;
             ++loaded;

@@ -146,6 +146,10 @@ typedef struct ssw_tuning {
     int32_t chunk;            /* automata taken per work-queue fetch */
     int32_t arena_lds;        /* frontier entries kept in LDS per automaton */
     int32_t lane_entries;     /* frontier entries per lane per level step: 1, 2 or 4 */
+    int32_t spill_arena_lds;  /* n <= 16: 0 = a frontier that outgrows the LDS ring moves to HBM;
+                                 else such automata are re-solved by a second launch whose LDS
+                                 ring holds this many entries */
+    int32_t spill_lane_entries; /* frontier entries per lane in that second launch: 1, 2 or 4 */
 } ssw_tuning;
 
 int ssw_get_tuning(ssw_context *ctx, int32_t n, ssw_tuning *out);
