@@ -24,9 +24,17 @@ def main():
     ap.add_argument("--out", default=os.path.join(HERE, "tuning_gfx950.json"))
     ap.add_argument("--n16-on-workload", action="store_true",
                     help="tune n=16 on the 1M-automaton n=16 workload with its 1%% Cerny share")
+    ap.add_argument("--start", default=None,
+                    help="tuning table to start from (default: the library's built-in defaults)")
+    ap.add_argument("--skip-random", action="store_true",
+                    help="keep the starting table for every n; only the workload tuning runs")
     args = ap.parse_args()
     with ssw.Context(0) as ctx:
+        if args.start:
+            ctx.load_tuning(args.start)
         for n in range(args.lo, args.hi + 1):
+            if args.skip_random:
+                break
             t0 = time.perf_counter()
             ctx.autotune(n, n, args.samples, args.seed, None)
             print(f"n={n}: {ctx.get_tuning(n)} ({time.perf_counter() - t0:.1f} s)", flush=True)

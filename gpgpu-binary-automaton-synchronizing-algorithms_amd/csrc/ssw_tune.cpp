@@ -86,10 +86,15 @@ int tune_one(ssw_context &ctx, int n, const DeviceBatch &db, Timed *out) {
     if (small) {
         // ring spill: automata that outgrow a small first-launch ring go to a
         // second launch with a large ring; then the first ring can shrink
-        if ((rc = sweep(ctx, n, db, best, {0, 1024, 2048, 4096, 8192}, [](ssw_tuning &t, int v) {
-                 t.spill_arena_lds = v;
-                 if (v && !t.spill_lane_entries) t.spill_lane_entries = 4;
-             })))
+        // joint moves (spill ring, first ring): spilling only pays with a
+        // small first ring, which alone would move wide frontiers to HBM
+        if ((rc = sweep(ctx, n, db, best, {0, 4096 * 8192 + 0, 4096 * 8192 + 256, 4096 * 8192 + 512,
+                                           4096 * 8192 + 1024, 2048 * 8192 + 512, 8192 * 8192 + 512},
+                        [](ssw_tuning &t, int v) {
+                            t.spill_arena_lds = v / 8192;
+                            if (v % 8192) t.arena_lds = v % 8192;
+                            if (t.spill_arena_lds && !t.spill_lane_entries) t.spill_lane_entries = 4;
+                        })))
             return rc;
         if (best.t.spill_arena_lds > 0) {
             if ((rc = sweep(ctx, n, db, best, {128, 256, 512, 1024},
