@@ -148,6 +148,9 @@ struct WaveGroup {
         }
         total = b - base;
     }
+    // bit k set when work-queue segment k still has work: the group's lanes
+    // read the kQueues segments in parallel
+    __device__ uint64_t live_segments(const uint32_t *queues) const;
     __device__ uint32_t bcast0(uint32_t v) const {
         if (G == 64) return __builtin_amdgcn_readfirstlane(v);
         return __shfl(v, (int)gbase, 64);
@@ -302,6 +305,8 @@ struct BlockGroup {
         total = __builtin_amdgcn_readfirstlane(b - base);
         any_single = __builtin_amdgcn_readfirstlane(anys ? 1u : 0u) != 0u;
     }
+    // wave 0 reads the kQueues segments in parallel
+    __device__ uint64_t live_segments(const uint32_t *queues) const;
     __device__ uint32_t bcast0(uint32_t v) const {
         __syncthreads();
         if (tid == 0) scratch[2 * kWaves] = v;
@@ -312,41 +317,72 @@ struct BlockGroup {
 
 // Device work queue: the batch is cut into kQueues contiguous segments, each
 // with its own counter on its own 128-byte line, so the atomics of thousands
-// of groups spread over many addresses; a group starts at segment
-// (slot mod kQueues) and moves on when its segment runs dry, recording the
-// dry segment in a shared mask that the others skip.
+// of groups spread over many addresses.  A group starts at segment
+// (slot mod kQueues); when that runs dry it reads all segments' (next, end)
+// pairs at once, one per lane (one round trip, distinct addresses), and moves
+// to the nearest live one.  A failed claim means that segment is dry for
+// good, so the loop ends after at most kQueues + 1 claims; once every
+// segment is dry a group leaves after one probe.
 template <class Grp>
 __device__ int64_t fetch_work(const BatchParams &P, const Grp &g, uint32_t &q, int64_t &it, int64_t &it_end) {
     if (it < it_end) return it++;
-    uint32_t base = 0xFFFFFFFFu, end = 0;
-    if (g.rank() == 0) {
-        unsigned long long *mask = (unsigned long long *)(P.queues + kQueues * kQueueWords);
-        for (int tries = 0; tries < kQueues; ++tries) {
-            const unsigned long long m = __hip_atomic_load(mask, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (m == ~0ull) break;
-            if ((m >> q) & 1ull) {
-                // next live segment after q (cyclically)
-                const unsigned long long live = ~m;
-                const unsigned long long hi_part = q + 1 < 64 ? (live >> (q + 1)) << (q + 1) : 0ull;
-                q = hi_part ? (uint32_t)__ffsll((long long)hi_part) - 1u : (uint32_t)__ffsll((long long)live) - 1u;
-            }
+    for (int tries = 0; tries <= kQueues; ++tries) {
+        uint32_t base = 0xFFFFFFFFu, end = 0;
+        if (g.rank() == 0) {
             const uint32_t b = atomicAdd(P.queues + q * kQueueWords, P.chunk);
             const uint32_t e = P.queues[q * kQueueWords + 1];
             if (b < e) {
                 base = b;
                 end = b + P.chunk < e ? b + P.chunk : e;
-                break;
             }
-            atomicOr(mask, 1ull << q);
-            q = (q + 1) & (kQueues - 1);
         }
+        base = g.bcast0(base);
+        end = g.bcast0(end);
+        if (base != 0xFFFFFFFFu) {
+            it = base;
+            it_end = end;
+            return it++;
+        }
+        const uint64_t live = g.live_segments(P.queues);
+        if (live == 0) return -1;
+        // nearest live segment after q, cyclically
+        const uint64_t above = q + 1 < 64 ? (live >> (q + 1)) << (q + 1) : 0ull;
+        q = above ? (uint32_t)__ffsll((long long)above) - 1u : (uint32_t)__ffsll((long long)live) - 1u;
     }
-    base = g.bcast0(base);
-    end = g.bcast0(end);
-    if (base == 0xFFFFFFFFu) return -1;
-    it = base;
-    it_end = end;
-    return it++;
+    return -1;
+}
+
+// (next, end) of work-queue segment k, read at device scope
+__device__ __forceinline__ bool segment_live(const uint32_t *queues, uint32_t k) {
+    const uint32_t nx = __hip_atomic_load(queues + k * kQueueWords, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t e = __hip_atomic_load(queues + k * kQueueWords + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return nx < e;
+}
+
+template <int G>
+__device__ uint64_t WaveGroup<G>::live_segments(const uint32_t *queues) const {
+    uint64_t live = 0;
+#pragma unroll
+    for (int j = 0; j < kQueues / G; ++j) {
+        const uint64_t b = __ballot(segment_live(queues, rank_ + j * G)) & gmask;
+        live |= ((b >> gbase) & (G == 64 ? ~0ull : ((1ull << (G & 63)) - 1ull))) << (j * G);
+    }
+    return G == 64 ? (uint64_t)uni<64>((int64_t)live) : live;
+}
+
+template <int NT>
+__device__ uint64_t BlockGroup<NT>::live_segments(const uint32_t *queues) const {
+    uint64_t live = 0;
+    if (wave == 0) live = __ballot(segment_live(queues, lane));
+    __syncthreads();
+    if (tid == 0) {
+        scratch[48] = (uint32_t)live;
+        scratch[49] = (uint32_t)(live >> 32);
+    }
+    __syncthreads();
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(scratch[48]);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(scratch[49]);
+    return ((uint64_t)hi << 32) | lo;
 }
 
 // Sets up the segments of a batch of `count` work items (read from
@@ -900,10 +936,12 @@ struct SmallSolver {
     }
 
     __device__ int step() {
-        // entries per lane follow the width left in the level (wide C_n levels
-        // take T per lane, narrow random-automaton levels one)
+        // entries per lane: the fewest steps for the width left in the level
+        // (a step's instruction count grows slower than its entries, so one
+        // T = 4 step over 150..256 entries costs less than T = 2 plus T = 1
+        // steps, and it saves their LDS round trips)
         const uint32_t left = hi - rd;
-        const uint32_t tt = (T >= 4 && left >= 4u * G) ? 4u : ((T >= 2 && left >= 2u * G) ? 2u : 1u);
+        const uint32_t tt = (T >= 4 && left > 2u * G) ? 4u : ((T >= 2 && left > G) ? 2u : 1u);
         // the ring holds the unread window plus two images per entry taken
         if (in_lds && wr + 2u * G * tt - rd > rcap) {
             if (P.spill) return spill();
@@ -1111,6 +1149,7 @@ static hipError_t occ_t(int block, size_t lds, int *blocks) {
     case V * 1000 + 16: return FN<SmallSolver<WaveGroup<16>, TT, BT>, WaveGroup<16>>(__VA_ARGS__);            \
     case V * 1000 + 32: return FN<SmallSolver<WaveGroup<32>, TT, BT>, WaveGroup<32>>(__VA_ARGS__);            \
     case V * 1000 + 64: return FN<SmallSolver<WaveGroup<64>, TT, BT>, WaveGroup<64>>(__VA_ARGS__);            \
+    case V * 1000 + 128: return FN<SmallSolver<BlockGroup<128>, TT, BT>, BlockGroup<128>>(__VA_ARGS__);       \
     case V * 1000 + 256: return FN<SmallSolver<BlockGroup<256>, TT, BT>, BlockGroup<256>>(__VA_ARGS__);
 #define SSW_DISPATCH(FN, ...)                                                              \
     switch (variant * 1000 + group_lanes) {                                                \
@@ -1137,7 +1176,7 @@ hipError_t batch_occupancy(int variant, int group_lanes, int block, size_t lds, 
 }
 
 size_t batch_scratch_bytes(int group_lanes) {
-    return group_lanes == 256 ? (size_t)BlockGroup<256>::kScratchBytes : 0;
+    return group_lanes > 64 ? (size_t)BlockGroup<256>::kScratchBytes : 0;
 }
 
 // LDS bytes of one workgroup of `groups` groups (with the group scratch), and

@@ -209,7 +209,8 @@ int ssw_context::launch_class(const DeviceBatch &db, const ClassSpec &cls, bool 
     int rc = launch_pass(db, cls, t, false, kPassSpillFrom);
     if (rc) return rc;
     ssw_tuning t2 = t;
-    t2.block_threads = 0;
+    if (t.spill_group_lanes > 0) t2.group_lanes = t.spill_group_lanes;
+    t2.block_threads = t2.group_lanes > 64 ? t2.group_lanes : 0;
     t2.chunk = 1;
     t2.arena_lds = t.spill_arena_lds;
     t2.lane_entries = t.spill_lane_entries > 0 ? t.spill_lane_entries : t.lane_entries;
@@ -218,7 +219,7 @@ int ssw_context::launch_class(const DeviceBatch &db, const ClassSpec &cls, bool 
 
 int ssw_context::launch_pass(const DeviceBatch &db, const ClassSpec &cls, const ssw_tuning &t, bool plain, int pass) {
     const int G = t.group_lanes;
-    int block = G == 256 ? 256 : std::max(64, std::min(256, t.block_threads ? t.block_threads : 256));
+    int block = G > 64 ? G : std::max(64, std::min(256, t.block_threads ? t.block_threads : 256));
     const bool use_lds = cls.lds;
     const int nmax = cls.nmax;
     const bool want_wit = db.witness != nullptr;
@@ -277,7 +278,7 @@ int ssw_context::launch_pass(const DeviceBatch &db, const ClassSpec &cls, const 
     // workgroup size: the tuned one, or (block_threads = 0) the one that keeps
     // the most groups resident per CU under the LDS budget
     int per_cu = 0;
-    if (!plain && G < 256 && t.block_threads == 0) {
+    if (!plain && G <= 64 && t.block_threads == 0) {
         int best_groups = 0;
         for (int cand = 64; cand <= 256; cand += 64) {
             if (cand % G) continue;
@@ -293,7 +294,7 @@ int ssw_context::launch_pass(const DeviceBatch &db, const ClassSpec &cls, const 
         }
     } else {
         // fewer groups per workgroup when the tuned block does not fit the LDS limit
-        while (G < 256 && block > 64 && batch_block_lds(variant, G, block / G, bm_words, arena_lds) > lds_per_block)
+        while (G <= 64 && block > 64 && batch_block_lds(variant, G, block / G, bm_words, arena_lds) > lds_per_block)
             block -= 64;
     }
     const int groups_per_block = block / G;
@@ -821,6 +822,9 @@ int ssw_set_tuning(ssw_context *ctx, int32_t n, const ssw_tuning *t) {
     if (t->spill_lane_entries != 0 && t->spill_lane_entries != 1 && t->spill_lane_entries != 2 &&
         t->spill_lane_entries != 4)
         return ctx->fail(SSW_ERR_ARG, "spill_lane_entries %d", t->spill_lane_entries);
+    const int SG = t->spill_group_lanes;
+    if (SG != 0 && SG != 16 && SG != 32 && SG != 64 && SG != 128 && SG != 256)
+        return ctx->fail(SSW_ERR_ARG, "spill_group_lanes %d", SG);
     ctx->tune[n] = *t;
     return SSW_OK;
 }

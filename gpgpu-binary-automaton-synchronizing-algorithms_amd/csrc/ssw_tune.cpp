@@ -41,6 +41,8 @@ bool valid_config(const ssw_tuning &t) {
     if (G != 16 && G != 32 && G != 64 && G != 256) return false;
     if (t.block_threads && (t.block_threads % 64 || t.block_threads > 256 || t.block_threads % G)) return false;
     if (G == 256 && t.block_threads && t.block_threads != 256) return false;
+    const int SG = t.spill_group_lanes;
+    if (SG != 0 && SG != 16 && SG != 32 && SG != 64 && SG != 128 && SG != 256) return false;
     return t.chunk >= 1 && t.arena_lds >= 4 && t.spill_arena_lds >= 0 && t.spill_lane_entries >= 0;
 }
 
@@ -104,6 +106,11 @@ int tune_one(ssw_context &ctx, int n, const DeviceBatch &db, Timed *out) {
                 return rc;
             if ((rc = sweep(ctx, n, db, best, {1, 2, 4}, [](ssw_tuning &t, int v) { t.spill_lane_entries = v; })))
                 return rc;
+            if ((rc = sweep(ctx, n, db, best, {0, 128, 256}, [](ssw_tuning &t, int v) { t.spill_group_lanes = v; })))
+                return rc;
+            if (best.t.spill_group_lanes > 64 &&
+                (rc = sweep(ctx, n, db, best, {1, 2, 4}, [](ssw_tuning &t, int v) { t.spill_lane_entries = v; })))
+                return rc;
         }
     }
     *out = best;
@@ -119,9 +126,10 @@ void write_json(FILE *f, ssw_context &ctx, int n_lo, int n_hi, int64_t samples, 
         const ssw_tuning &t = ctx.tune[n];
         fprintf(f,
                 "    \"%d\": {\"group_lanes\": %d, \"block_threads\": %d, \"chunk\": %d, \"arena_lds\": %d, "
-                "\"lane_entries\": %d, \"spill_arena_lds\": %d, \"spill_lane_entries\": %d, \"ms\": %.4f}%s\n",
+                "\"lane_entries\": %d, \"spill_arena_lds\": %d, \"spill_lane_entries\": %d, "
+                "\"spill_group_lanes\": %d, \"ms\": %.4f}%s\n",
                 n, t.group_lanes, t.block_threads, t.chunk, t.arena_lds, t.lane_entries, t.spill_arena_lds,
-                t.spill_lane_entries, ms[n - n_lo], n < n_hi ? "," : "");
+                t.spill_lane_entries, t.spill_group_lanes, ms[n - n_lo], n < n_hi ? "," : "");
     }
     fprintf(f, "  }\n}\n");
 }
@@ -233,6 +241,7 @@ int ssw_load_tuning(ssw_context *ctx, const char *path) {
         read_int(obj, "lane_entries", &t.lane_entries);
         read_int(obj, "spill_arena_lds", &t.spill_arena_lds);
         read_int(obj, "spill_lane_entries", &t.spill_lane_entries);
+        read_int(obj, "spill_group_lanes", &t.spill_group_lanes);
         if (n >= 1 && n <= SSW_MAX_N && valid_config(t)) {
             ctx->tune[n] = t;
             ++loaded;

@@ -150,6 +150,8 @@ typedef struct ssw_tuning {
                                  else such automata are re-solved by a second launch whose LDS
                                  ring holds this many entries */
     int32_t spill_lane_entries; /* frontier entries per lane in that second launch: 1, 2 or 4 */
+    int32_t spill_group_lanes;  /* threads per automaton in that second launch: 0 (= group_lanes),
+                                   16, 32, 64, 128 or 256 */
 } ssw_tuning;
 
 int ssw_get_tuning(ssw_context *ctx, int32_t n, ssw_tuning *out);
