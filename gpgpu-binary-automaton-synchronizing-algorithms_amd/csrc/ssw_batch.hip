@@ -756,7 +756,7 @@ struct SmallSolver {
     static constexpr bool kNestedLoop = G >= 64;
     const BatchParams &P;
     Grp g;
-    uint32_t tab_a, stage_a, pimg_a, ring_a, bm_a;  // LDS byte addresses
+    uint32_t tab_a, stage_a, pimg_a, ctl_a, ring_a, bm_a;  // LDS byte addresses
     glb_u16 *gring;
     uint32_t rmask, rcap;
     bool in_lds;
@@ -784,7 +784,8 @@ struct SmallSolver {
         tab_a = base + SmallLds::regions_base(groups, bm_bytes) + gib * P.lds_group_bytes;
         stage_a = tab_a + SmallLds::tab_bytes(BT);
         pimg_a = stage_a + 64;
-        ring_a = pimg_a + 64;
+        ctl_a = pimg_a + 64;
+        ring_a = ctl_a + 64;
         gring = (glb_u16 *)(slot + P.garena_off);
         rcap = P.lds_arena_cap;
         rmask = rcap - 1u;
@@ -920,6 +921,10 @@ struct SmallSolver {
         if (r == 0) {
             or_wg(l32(bm_a | ((Q >> 3) & 0x1FFCu)), 1u << (Q & 31u));
             *l16(ring_a) = (uint16_t)Q;
+            if constexpr (G > 64) {
+#pragma unroll
+                for (uint32_t k = 0; k < 6u; ++k) *l32(ctl_a + 4u * k) = 0u;
+            }
         }
         lsync();
         hi = 1;
@@ -936,6 +941,7 @@ struct SmallSolver {
     }
 
     __device__ int step() {
+        if constexpr (G > 64) return step_level();
         // entries per lane: the fewest steps for the width left in the level
         // (a step's instruction count grows slower than its entries, so one
         // T = 4 step over 150..256 entries costs less than T = 2 plus T = 1
@@ -956,6 +962,135 @@ struct SmallSolver {
             if (tt == 2u) return in_lds ? step_in<true, 2>() : step_in<false, 2>();
         }
         return in_lds ? step_in<true, 1>() : step_in<false, 1>();
+    }
+
+    // ---- groups of several wavefronts (G > 64) ----
+    // The waves split each level into chunks of 64 * tt entries (wave w takes
+    // chunks w, w + W, ...), append their new subsets to the shared ring after
+    // one LDS atomicAdd per chunk on the level's counter, and meet at one
+    // barrier per level.  The counters (ctl words 0..2) and found flags (3..5)
+    // rotate over three slots: slot l % 3 serves level l, and slot
+    // (l + 2) % 3 is cleared after level l's barrier, when every wave has read
+    // it for level l - 1.  A level whose entries plus all the images they can
+    // add do not fit the LDS ring is run on the HBM ring (copied there and
+    // back at level boundaries).
+    static constexpr uint32_t kW = G / 64;
+    __device__ __forceinline__ lds_u32 *ctl(uint32_t k) const { return l32(ctl_a + 4u * k); }
+
+    __device__ void relocate(bool to_lds) {
+        for (uint32_t p = rd + g.rank(); p < hi; p += G) {
+            if (to_lds)
+                *l16(ring_a + ((p & rmask) << 1)) = gring[p & (kGlobalRing - 1u)];
+            else
+                gring[p & (kGlobalRing - 1u)] = (uint16_t)rget<true>(p);
+        }
+        g.sync();
+        in_lds = to_lds;
+    }
+
+    __device__ int step_level() {
+        const uint32_t lo = rd, width = hi - lo;
+        const bool fits = 3u * width <= rcap;
+        if (in_lds != fits) {
+            if (!fits && P.spill) return spill();
+            relocate(fits);
+        }
+        const uint32_t share = (width + kW - 1u) / kW;
+        const uint32_t tt = (T >= 4 && share > 128u) ? 4u : ((T >= 2 && share > 64u) ? 2u : 1u);
+        const uint32_t slot = level % 3u;
+        const uint32_t wave = g.wave;
+        bool single = false;
+        for (uint32_t p0 = lo + wave * 64u * tt; p0 < hi; p0 += kW * 64u * tt) {
+            if constexpr (T >= 4) {
+                if (tt == 4u) {
+                    single |= in_lds ? chunk<true, 4>(p0, slot) : chunk<false, 4>(p0, slot);
+                    continue;
+                }
+            }
+            if constexpr (T >= 2) {
+                if (tt == 2u) {
+                    single |= in_lds ? chunk<true, 2>(p0, slot) : chunk<false, 2>(p0, slot);
+                    continue;
+                }
+            }
+            single |= in_lds ? chunk<true, 1>(p0, slot) : chunk<false, 1>(p0, slot);
+        }
+        if (single && g.lane == 0) *ctl(3u + slot) = 1u;
+        g.sync();
+        const uint32_t added = __builtin_amdgcn_readfirstlane(*ctl(slot));
+        const bool found = __builtin_amdgcn_readfirstlane(*ctl(3u + slot)) != 0u;
+        if (g.rank() == 0) {
+            const uint32_t c = (level + 2u) % 3u;
+            *ctl(c) = 0u;
+            *ctl(3u + c) = 0u;
+        }
+        wr = hi + added;
+        if (found) return kFound;
+        if (added == 0u) return kNoSync;
+        ++level;
+        rd = hi;
+        hi = wr;
+        return kContinue;
+    }
+
+    // One wave's chunk of 64 * TT entries from p0 (p0 < hi); true when an
+    // image is a singleton (uniform over the wave).
+    template <bool LDS, int TT>
+    __device__ bool chunk(uint32_t p0, uint32_t slot) {
+        const uint32_t lane = g.lane;
+        uint32_t s[TT];
+        bool v[TT];
+#pragma unroll
+        for (int t = 0; t < TT; ++t) {
+            const uint32_t p = p0 + t * 64u + lane;
+            v[t] = p < hi;
+            s[t] = rget<LDS>(v[t] ? p : p0);
+        }
+        uint32_t img[TT];
+#pragma unroll
+        for (int t = 0; t < TT; ++t) img[t] = image(s[t]);
+        bool single = false;
+        bool nw[2 * TT];
+#pragma unroll
+        for (int t = 0; t < TT; ++t) {
+            const uint32_t x = img[t];
+            const uint32_t y = x & (x - 0x00010001u);
+            single |= ((uint16_t)y == 0) | (y < 0x10000u);
+            const uint32_t ba = 1u << (x & 31u), bb = 1u << ((x >> 16) & 31u);
+            uint32_t oa = ~0u, ob = ~0u;
+            if (v[t]) {
+                oa = or_wg(l32(bm_a | ((x >> 3) & 0x1FFCu)), ba);
+                ob = or_wg(l32(bm_a | ((x >> 19) & 0x1FFCu)), bb);
+            }
+            nw[2 * t] = (oa & ba) == 0u;
+            nw[2 * t + 1] = (ob & bb) == 0u;
+        }
+        // wave-local flag-major positions, then one reservation on the
+        // level's counter
+        uint32_t pos[2 * TT], total = 0;
+#pragma unroll
+        for (int j = 0; j < 2 * TT; ++j) {
+            const uint64_t m = __ballot(nw[j]);
+            pos[j] = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, total));
+            total += (uint32_t)__popcll(m);
+        }
+        total = __builtin_amdgcn_readfirstlane(total);
+        if (total) {
+            uint32_t base = 0;
+            if (lane == 0) base = __hip_atomic_fetch_add(ctl(slot), total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            base = hi + __builtin_amdgcn_readfirstlane(base);
+#pragma unroll
+            for (int j = 0; j < 2 * TT; ++j) {
+                const uint32_t val = (j & 1) ? (img[j >> 1] >> 16) : (img[j >> 1] & 0xFFFFu);
+                if constexpr (LDS) {
+                    const uint32_t q = nw[j] ? ((base + pos[j]) & rmask) : (rcap + lane);
+                    *l16(ring_a + (q << 1)) = (uint16_t)val;
+                } else {
+                    if (nw[j]) gring[(base + pos[j]) & (kGlobalRing - 1u)] = (uint16_t)val;
+                }
+            }
+        }
+        return __ballot(single) != 0;
     }
 
     // Hands the automaton to the second launch (larger LDS ring).

@@ -26,8 +26,9 @@ enum StepCode : int { kContinue = 0, kFound = 1, kNoSync = 2, kOverflow = 3, kSp
 // groups first, each aligned to its size (a power of two, at least 256
 // bytes), then one region per group -- image table (four nibble slices,
 // 256 bytes, or two byte slices, 2 KiB), transitions staging (64 bytes),
-// singleton images of the states (64 bytes), and the frontier ring of
-// ring_cap u16 entries plus 64 dummy entries -- then the group scratch.
+// singleton images of the states (64 bytes), control words of groups of
+// several wavefronts (64 bytes), and the frontier ring of ring_cap u16
+// entries plus 64 dummy entries -- then the group scratch.
 struct SmallLds {
     static __host__ __device__ inline uint32_t bitmap_bytes(int n) {
         const uint32_t b = n >= 3 ? (1u << (n - 3)) : 1u;
@@ -35,7 +36,7 @@ struct SmallLds {
     }
     static __host__ __device__ inline uint32_t tab_bytes(bool byte_tab) { return byte_tab ? 2048u : 256u; }
     static __host__ __device__ inline uint32_t region_bytes(bool byte_tab, uint32_t ring_cap) {
-        const uint32_t b = tab_bytes(byte_tab) + 64u + 64u + 2u * (ring_cap + 64u);
+        const uint32_t b = tab_bytes(byte_tab) + 64u + 64u + 64u + 2u * (ring_cap + 64u);
         const uint32_t a = byte_tab ? 1024u : 256u;
         return (b + a - 1u) / a * a;
     }

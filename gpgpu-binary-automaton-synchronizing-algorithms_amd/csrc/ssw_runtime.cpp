@@ -265,8 +265,10 @@ int ssw_context::launch_pass(const DeviceBatch &db, const ClassSpec &cls, const 
     uint32_t arena_lds;
     if (small) {
         // a ring holding every subset of Q never has to leave LDS; a spilling
-        // pass only needs room for one narrow step
-        arena_lds = next_pow2((uint64_t)std::max(t.arena_lds, pass == kPassSpillFrom ? 4 * G : 4 * G * T));
+        // pass only needs room for one narrow step; groups of several
+        // wavefronts check whole levels against the ring instead of steps
+        const int min_ring = G > 64 ? 256 : (pass == kPassSpillFrom ? 4 * G : 4 * G * T);
+        arena_lds = next_pow2((uint64_t)std::max(t.arena_lds, min_ring));
         arena_lds = std::min(arena_lds, next_pow2(all_subsets + 2ull * G * T));
     } else {
         arena_lds = (uint32_t)align_up((uint64_t)std::max(4, t.arena_lds), 4);

@@ -1,0 +1,109 @@
+"""GPU parity of the n <= 16 path's launch configurations against the CPU oracle
+(restatement of SlimCPUUntweaked.cs:51-222): ring spill to a second launch,
+groups of one or several wavefronts, frontiers that move between the LDS ring
+and the HBM ring, lane-entry counts 1/2/4, nibble and byte image tables.
+
+Every configuration must give the same lengths, flags and canonical expanded
+counts, bit for bit.  The batch mixes Cerny C_16 in both letter conventions
+(wide frontiers, 225 levels), the widest random automata of a larger sample
+(frontiers of several hundred subsets) and ordinary random automata.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import ssw
+import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+N = 16
+
+
+def _batch():
+    _, delta, _ = W.uniform_n16(count=40_000, cerny_every=0)
+    A = delta[:, :N].astype(np.int32)
+    B = delta[:, N:].astype(np.int32)
+    L, S, E, _ = O.solve(N, A, B, threads=8)
+    wide = np.argsort(E)[-300:]             # the random automata with the most subsets
+    plain = np.arange(3000)
+    rows = [A[plain], A[wide]]
+    cols = [B[plain], B[wide]]
+    for conv in (0, 1):
+        a, b = ssw.generate_cerny(N, conv)
+        rows.append(np.tile(a, (40, 1)))
+        cols.append(np.tile(b, (40, 1)))
+    A2 = np.concatenate(rows)
+    B2 = np.concatenate(cols)
+    perm = np.random.default_rng(11).permutation(len(A2))
+    return A2[perm], B2[perm]
+
+
+@pytest.fixture(scope="module")
+def case():
+    A, B = _batch()
+    L, S, E, _ = O.solve(N, A, B, threads=8)
+    return A, B, L, S, E
+
+
+# (group_lanes, block_threads, chunk, arena_lds, lane_entries, spill_arena_lds, spill_lane_entries,
+#  spill_group_lanes)
+CONFIGS = [
+    (64, 128, 16, 512, 1, 4096, 4, 0),      # the committed n = 16 configuration
+    (64, 0, 4, 256, 1, 4096, 2, 0),
+    (64, 192, 8, 512, 2, 2048, 4, 0),       # spill ring too small for C_16: HBM ring in the second launch
+    (64, 128, 16, 512, 1, 4096, 4, 128),    # two wavefronts per automaton in the second launch
+    (64, 128, 16, 512, 1, 4096, 1, 256),
+    (64, 128, 16, 512, 1, 1024, 4, 256),    # wide levels relocate to the HBM ring and back
+    (32, 0, 8, 512, 1, 4096, 4, 0),         # two automata per wavefront in the first launch
+    (256, 256, 2, 512, 2, 4096, 4, 0),      # four wavefronts per automaton, spilling from the first launch
+    (256, 256, 2, 1024, 4, 0, 0, 0),        # no spill: level-split groups with HBM relocation
+    (64, 0, 2, 2048, 4, 0, 0, 0),           # no spill: one wavefront, frontier moves to HBM
+]
+
+
+@pytest.mark.parametrize("cfg", CONFIGS, ids=[",".join(map(str, c)) for c in CONFIGS])
+def test_configuration_parity(ctx, case, cfg):
+    A, B, L, S, E = case
+    saved = ctx.get_tuning(N)
+    try:
+        ctx.set_tuning(N, *cfg)
+        print(f"config {cfg}", flush=True)
+        r = ctx.compute(N, A, B)
+    finally:
+        ctx.set_tuning(N, **saved)
+    bad = np.nonzero(r.length != L)[0]
+    assert len(bad) == 0, f"{cfg}: length mismatch at {bad[:10]} gpu={r.length[bad[:10]]} cpu={L[bad[:10]]}"
+    np.testing.assert_array_equal(r.sync, S, err_msg=f"{cfg}: sync")
+    np.testing.assert_array_equal(r.expanded, E, err_msg=f"{cfg}: expanded")
+
+
+def test_spill_mixed_sizes(ctx):
+    """Uneven sizes with spill on for n = 12..16: every class spills into its
+    own list and second launch, concurrently with the others."""
+    rng = np.random.default_rng(23)
+    ns = rng.integers(12, 17, size=4000).astype(np.uint8)
+    As, Bs = [], []
+    for i, n in enumerate(ns):
+        if i % 97 == 0:
+            a, b = ssw.generate_cerny(int(n), i % 2)
+        else:
+            a = rng.integers(0, n, size=n)
+            b = rng.integers(0, n, size=n)
+        As.append(np.asarray(a, np.int32))
+        Bs.append(np.asarray(b, np.int32))
+    saved = {n: ctx.get_tuning(n) for n in range(12, 17)}
+    try:
+        for n in range(12, 17):
+            ctx.set_tuning(n, 64, 0, 8, 256, 1, 4096, 4, 0)
+        r = ctx.solve_mixed(ns, As, Bs)
+    finally:
+        for n, t in saved.items():
+            ctx.set_tuning(n, **t)
+    for n in range(12, 17):
+        idx = np.nonzero(ns == n)[0]
+        A = np.stack([As[i] for i in idx])
+        B = np.stack([Bs[i] for i in idx])
+        L, S, E, _ = O.solve(n, A, B, threads=8)
+        np.testing.assert_array_equal(r.length[idx], L, err_msg=f"n={n}")
+        np.testing.assert_array_equal(r.expanded[idx], E, err_msg=f"n={n}")
