@@ -38,7 +38,8 @@ def _compile(src: str) -> str:
     obj = os.path.join(OBJ, src + ".o")
     cmd = [hipcc(), *COMMON, *DEVICE, "-c", path, "-o", obj]
     if src.endswith(".cpp"):
-        cmd = [hipcc(), *COMMON, "-c", path, "-o", obj]
+        # host code: no device code object other than gfx950's (empty)
+        cmd = [hipcc(), *COMMON, f"--offload-arch={ARCH}", "-c", path, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

@@ -385,15 +385,15 @@ __device__ uint64_t BlockGroup<NT>::live_segments(const uint32_t *queues) const 
     return ((uint64_t)hi << 32) | lo;
 }
 
-// Sets up the segments of a batch of `count` work items (read from
-// count_dev when it is not null).
-__global__ void init_queues_kernel(uint32_t *queues, int64_t count, const uint32_t *count_dev) {
-    if (count_dev) count = (int64_t)*count_dev;
+// Sets up the segments of a batch of `count` work items, or (empty = 1) the
+// same segments with no items yet, for a launch whose items are appended by
+// the one before it.
+__global__ void init_queues_kernel(uint32_t *queues, int64_t count, uint32_t empty) {
     const int k = threadIdx.x;
     if (k < kQueues) {
         const uint32_t s = (uint32_t)(count * k / kQueues), e = (uint32_t)(count * (k + 1) / kQueues);
         queues[k * kQueueWords] = s;
-        queues[k * kQueueWords + 1] = e;
+        queues[k * kQueueWords + 1] = empty ? s : e;
     }
     __syncthreads();
     if (k == 0) {
@@ -760,6 +760,7 @@ struct SmallSolver {
     glb_u16 *gring;
     uint32_t rmask, rcap;
     bool in_lds;
+    bool narrow;  // relabelled states, visited bitmap of 2^narrow_bits bits
     int64_t ai;
     int32_t n;
     uint32_t Q;
@@ -773,6 +774,7 @@ struct SmallSolver {
     int64_t nx_ai;
     int32_t nx_n;
     uint32_t nx_b0, nx_b1;
+    uint32_t nx_seg, seg;  // work-queue segment of the next / current automaton
 
     __device__ SmallSolver(const BatchParams &p, uint8_t *smem, uint32_t gib, uint32_t groups, uint8_t *slot)
         : P(p), g(smem + SmallLds::block_bytes(groups, 4u * p.lds_bitmap_words, p.lds_group_bytes, 0)) {
@@ -789,6 +791,7 @@ struct SmallSolver {
         gring = (glb_u16 *)(slot + P.garena_off);
         rcap = P.lds_arena_cap;
         rmask = rcap - 1u;
+        narrow = P.narrow_bits > 0;
         it = 0;
         it_end = 0;
         wq = 0;
@@ -850,6 +853,7 @@ struct SmallSolver {
             return;
         }
         nx_ai = uni<G>(P.order ? (int64_t)P.order[item] : item);
+        nx_seg = wq;
         nx_n = uni<G>(P.uniform_n > 0 ? P.uniform_n : (int32_t)P.n_per[nx_ai]);
         nx_b0 = nx_b1 = 0;
         if (nx_n >= 1 && nx_n <= P.lds_n) {
@@ -868,6 +872,7 @@ struct SmallSolver {
     __device__ int begin() { return start(); }
     __device__ int start() {
         ai = nx_ai;
+        seg = nx_seg;
         n = uni<G>(nx_n);
         const uint32_t r = g.rank();
         const uint32_t b0 = nx_b0, b1 = nx_b1;
@@ -884,13 +889,39 @@ struct SmallSolver {
         }
         Q = uni<G>((1u << n) - 1u);
         lsync();
-        // singleton images of the 16 states, zero past n
-        if (r < 16) {
-            uint32_t pi = 0;
-            if (r < (uint32_t)n) pi = (1u << *l8(stage_a + r)) | (1u << (16u + *l8(stage_a + (uint32_t)n + r)));
-            *l32(pimg_a + 4u * r) = pi;
+        uint32_t qa = 0, qb = 0;  // the images of state r under a and b
+        if (r < (uint32_t)n) {
+            qa = *l8(stage_a + r);
+            qb = *l8(stage_a + (uint32_t)n + r);
         }
-        const uint32_t words = n >= 5 ? (1u << (n - 5)) : 1u;
+        uint32_t words;
+        if (narrow) {
+            // Every subset reached from Q but Q itself lies inside
+            // U = a(Q) | b(Q) (the image of any subset is inside the image of
+            // Q).  The states are relabelled so that U takes the |U| lowest
+            // labels; the visited bitmap then needs 2^|U| bits, and automata
+            // with |U| above the bitmap's width go to the next launch.
+            uint32_t m = r < (uint32_t)n ? (1u << qa) | (1u << qb) : 0u;
+#pragma unroll
+            for (int o = 8; o >= 1; o >>= 1) m |= (uint32_t)__shfl_xor((int)m, o, 64);
+            const uint32_t U = g.bcast0(m);
+            const uint32_t k = (uint32_t)__popc(U);
+            if (k > (uint32_t)P.narrow_bits) return spill();
+            const auto label = [U, k](uint32_t x) {
+                const uint32_t below = (uint32_t)__popc(U & ((1u << x) - 1u));
+                return ((U >> x) & 1u) ? below : k + x - below;
+            };
+            // singleton images of the relabelled states, zero past n
+            if (r < (uint32_t)n)
+                *l32(pimg_a + 4u * label(r)) = (1u << label(qa)) | (1u << (16u + label(qb)));
+            else if (r < 16u)
+                *l32(pimg_a + 4u * r) = 0u;
+            words = 1u << (P.narrow_bits - 5);
+        } else {
+            // singleton images of the 16 states, zero past n
+            if (r < 16u) *l32(pimg_a + 4u * r) = r < (uint32_t)n ? (1u << qa) | (1u << (16u + qb)) : 0u;
+            words = n >= 5 ? (1u << (n - 5)) : 1u;
+        }
         if (words >= 4) {
             for (uint32_t w = r; w < (words >> 2); w += G) *l128(bm_a + 16u * w) = u32x4{0u, 0u, 0u, 0u};
         } else {
@@ -919,7 +950,8 @@ struct SmallSolver {
         }
         lsync();
         if (r == 0) {
-            or_wg(l32(bm_a | ((Q >> 3) & 0x1FFCu)), 1u << (Q & 31u));
+            // relabelled, Q lies outside the narrow bitmap and is never reached again
+            if (!narrow) or_wg(l32(bm_a | ((Q >> 3) & 0x1FFCu)), 1u << (Q & 31u));
             *l16(ring_a) = (uint16_t)Q;
             if constexpr (G > 64) {
 #pragma unroll
@@ -1096,7 +1128,7 @@ struct SmallSolver {
     // Hands the automaton to the second launch (larger LDS ring).
     __device__ int spill() {
         if (g.rank() == 0) {
-            const uint32_t k = atomicAdd(P.spill_count, 1u);
+            const uint32_t k = atomicAdd(P.spill_queue + seg * kQueueWords + 1, 1u);
             P.spill_list[k] = (uint32_t)ai;
         }
         return kSpilled;
@@ -1328,8 +1360,8 @@ size_t batch_block_lds(int variant, int group_lanes, int groups, uint32_t bitmap
                                  (uint32_t)scratch);
 }
 
-hipError_t init_queues(uint32_t *queues, int64_t count, hipStream_t st, const uint32_t *count_dev) {
-    hipLaunchKernelGGL(init_queues_kernel, dim3(1), dim3(kQueues), 0, st, queues, count, count_dev);
+hipError_t init_queues(uint32_t *queues, int64_t count, hipStream_t st, bool empty) {
+    hipLaunchKernelGGL(init_queues_kernel, dim3(1), dim3(kQueues), 0, st, queues, count, empty ? 1u : 0u);
     return hipGetLastError();
 }
 

@@ -56,13 +56,24 @@ struct ClassSpec {
     int plain_n = 0;
     hipStream_t stream = nullptr;     // null: the context's stream
     uint32_t *queues = nullptr;       // this class's device work queue
-    uint32_t *spill_count = nullptr;  // ring-spill counter and list (null: no spill pass)
-    uint32_t *spill_list = nullptr;
+    uint32_t *spill_queue[2] = {nullptr, nullptr};  // work queues and lists of the later launches
+    uint32_t *spill_list[2] = {nullptr, nullptr};   // of a spill chain (null: no spill launches)
 };
 
 // launch passes of one class
 enum PassKind { kPassAll = 0, kPassSpillFrom = 1, kPassSpillTo = 2 };
-constexpr size_t kSpillCtrBytes = 256;
+
+// What one launch of a class reads and writes besides the batch.
+struct PassIO {
+    int kind = kPassAll;
+    const uint32_t *in_list = nullptr;  // work items: the class order, or this list ...
+    uint32_t *in_queue = nullptr;       // ... with this queue, filled by the launch before
+    uint32_t *out_list = nullptr;       // automata handed to the next launch
+    uint32_t *out_queue = nullptr;
+    int narrow_bits = 0;                // relabelled states, 2^narrow_bits-bit bitmap
+    bool byte_tab = false;
+};
+constexpr int kClassQueues = 3;  // work queues per launch class: its own and two spill queues
 
 struct SlotKey {
     int G, block, lds_n, nmax, witness, arena_lds, plain;
@@ -184,7 +195,8 @@ struct ssw_context {
     int ensure(ssw::DevBuf &b, size_t bytes, const char *what);
     int get_slots(const ssw::SlotKey &key, uint64_t bytes, ssw::DevBuf **out, hipStream_t st);
     int launch_class(const ssw::DeviceBatch &db, const ssw::ClassSpec &cls, bool plain);
-    int launch_pass(const ssw::DeviceBatch &db, const ssw::ClassSpec &cls, const ssw_tuning &t, bool plain, int pass);
+    int launch_pass(const ssw::DeviceBatch &db, const ssw::ClassSpec &cls, const ssw_tuning &t, bool plain,
+                    const ssw::PassIO &io);
     int solve_device(const ssw::DeviceBatch &db, const uint8_t *host_n, bool plain, int64_t *violation_out);
     int solve_host_packed(const ssw_batch *b, int mode, int64_t *violation_out);
     int solve_entry(const ssw_batch *b, int32_t memory, int mode, int64_t *violation_out);

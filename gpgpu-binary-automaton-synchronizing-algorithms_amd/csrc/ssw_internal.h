@@ -88,12 +88,19 @@ struct BatchParams {
     uint32_t single_item;              // plain path: solve only first_item
     int64_t first_item;
     // Ring spill (n <= 16 path): with spill = 1 an automaton whose frontier
-    // outgrows the LDS ring is not moved to HBM; its index is appended to
-    // spill_list (spill_count entries) and a second launch with a larger ring
-    // solves it.  The second launch reads its item count from spill_count.
+    // outgrows the LDS ring is not moved to HBM; it is handed to the next
+    // launch of the class.  spill_queue is that launch's work queue, set up
+    // with empty segments at the same boundaries as this launch's queue: an
+    // automaton taken from segment k is appended to segment k (atomicAdd on
+    // its end word, so appends spread over kQueues addresses) and its index
+    // stored in spill_list at that position.
     uint32_t spill;
-    uint32_t *spill_count;
+    uint32_t *spill_queue;
     uint32_t *spill_list;
+    // n <= 16 path: > 0 relabels every automaton's states so that
+    // U = a(Q) | b(Q) takes the lowest labels and uses a visited bitmap of
+    // 2^narrow_bits bits; automata with |U| > narrow_bits are spilled at start
+    int32_t narrow_bits;
 };
 
 // Device-wide path: every instance keeps its visited bitmap (2^n bits) and a

@@ -29,7 +29,7 @@ size_t batch_scratch_bytes(int group_lanes);
 size_t batch_group_lds(int variant, uint32_t bitmap_words, uint32_t arena_cap);
 size_t batch_block_lds(int variant, int group_lanes, int groups, uint32_t bitmap_words, uint32_t arena_cap);
 hipError_t fill_u32(uint32_t *p, uint64_t words, uint32_t value, hipStream_t st);
-hipError_t init_queues(uint32_t *queues, int64_t count, hipStream_t st, const uint32_t *count_dev);
+hipError_t init_queues(uint32_t *queues, int64_t count, hipStream_t st, bool empty);
 }  // namespace ssw
 
 using namespace ssw;
@@ -199,25 +199,60 @@ static inline uint32_t next_pow2(uint64_t x) {
 }
 
 // One launch class.  With ring spill on (n <= 16 path, spill_arena_lds > 0)
-// it is two launches on the class's stream: the tuned configuration, in which
-// automata whose frontier outgrows the small LDS ring are listed instead of
-// moved to HBM, then a launch with the larger ring over that list.
+// it is a chain of launches on the class's stream, each over the automata the
+// previous one listed:
+//   1. the tuned configuration; automata whose frontier outgrows the small LDS
+//      ring are listed instead of moved to HBM.  With narrow_bits, states are
+//      relabelled so that a(Q) | b(Q) takes the lowest labels and the visited
+//      bitmap has 2^narrow_bits bits; automata with |a(Q) | b(Q)| above that
+//      are listed at start,
+//   2. (narrow_bits only) the same ring with the full 2^n-bit bitmap,
+//   3. the larger ring (spill_arena_lds), byte tables.
 int ssw_context::launch_class(const DeviceBatch &db, const ClassSpec &cls, bool plain) {
-    const ssw_tuning t = plain ? ssw_tuning{256, 256, 1, 8192, 1, 0, 0} : tune[cls.nmax];
+    const ssw_tuning t = plain ? ssw_tuning{256, 256, 1, 8192, 1, 0, 0, 0, 0} : tune[cls.nmax];
     const bool small = cls.lds && db.witness == nullptr && cls.nmax <= kMaxLdsN;
-    if (plain || !small || t.spill_arena_lds <= 0 || !cls.spill_count) return launch_pass(db, cls, t, plain, kPassAll);
-    int rc = launch_pass(db, cls, t, false, kPassSpillFrom);
+    if (plain || !small || t.spill_arena_lds <= 0 || !cls.spill_queue[0])
+        return launch_pass(db, cls, t, plain, PassIO{});
+    const int nb = (t.narrow_bits >= 8 && t.narrow_bits < cls.nmax) ? t.narrow_bits : 0;
+    hipStream_t st = cls.stream ? cls.stream : stream;
+    // the later launches' queues: the first launch's segments, empty
+    for (int j = 0; j < (nb ? 2 : 1); ++j)
+        HIP_TRY(init_queues(cls.spill_queue[j], cls.items, st, true), "init queues");
+    PassIO first;
+    first.kind = kPassSpillFrom;
+    first.out_list = cls.spill_list[0];
+    first.out_queue = cls.spill_queue[0];
+    first.narrow_bits = nb;
+    int rc = launch_pass(db, cls, t, false, first);
     if (rc) return rc;
+    PassIO last;
+    last.kind = kPassSpillTo;
+    last.in_list = cls.spill_list[0];
+    last.in_queue = cls.spill_queue[0];
+    last.byte_tab = true;
+    if (nb) {
+        PassIO mid;
+        mid.kind = kPassSpillFrom;
+        mid.in_list = cls.spill_list[0];
+        mid.in_queue = cls.spill_queue[0];
+        mid.out_list = cls.spill_list[1];
+        mid.out_queue = cls.spill_queue[1];
+        rc = launch_pass(db, cls, t, false, mid);
+        if (rc) return rc;
+        last.in_list = cls.spill_list[1];
+        last.in_queue = cls.spill_queue[1];
+    }
     ssw_tuning t2 = t;
     if (t.spill_group_lanes > 0) t2.group_lanes = t.spill_group_lanes;
     t2.block_threads = t2.group_lanes > 64 ? t2.group_lanes : 0;
     t2.chunk = 1;
     t2.arena_lds = t.spill_arena_lds;
     t2.lane_entries = t.spill_lane_entries > 0 ? t.spill_lane_entries : t.lane_entries;
-    return launch_pass(db, cls, t2, false, kPassSpillTo);
+    return launch_pass(db, cls, t2, false, last);
 }
 
-int ssw_context::launch_pass(const DeviceBatch &db, const ClassSpec &cls, const ssw_tuning &t, bool plain, int pass) {
+int ssw_context::launch_pass(const DeviceBatch &db, const ClassSpec &cls, const ssw_tuning &t, bool plain,
+                             const PassIO &io) {
     const int G = t.group_lanes;
     int block = G > 64 ? G : std::max(64, std::min(256, t.block_threads ? t.block_threads : 256));
     const bool use_lds = cls.lds;
@@ -226,9 +261,9 @@ int ssw_context::launch_pass(const DeviceBatch &db, const ClassSpec &cls, const 
     // the small path: bitmap in LDS, no witness, 16-bit ring frontier
     const bool small = use_lds && !want_wit && nmax <= kMaxLdsN;
     const int T = t.lane_entries >= 4 ? 4 : (t.lane_entries >= 2 ? 2 : 1);
-    // the spill-to pass takes the wide searches: byte-sliced image tables
-    // (two lookups per subset instead of four)
-    const bool byte_tab = pass == kPassSpillTo;
+    // the last launch of a spill chain takes the wide searches: byte-sliced
+    // image tables (two lookups per subset instead of four)
+    const bool byte_tab = io.byte_tab;
     const int variant = small ? (T == 1 ? 1 : (T == 2 ? 2 : 3)) + (byte_tab ? 3 : 0) : 0;
 
     BatchParams P;
@@ -239,10 +274,11 @@ int ssw_context::launch_pass(const DeviceBatch &db, const ClassSpec &cls, const 
     P.n_per = db.n_per;
     P.offset = db.offset;
     P.delta = db.delta;
-    P.order = pass == kPassSpillTo ? cls.spill_list : cls.order;
-    P.spill = pass == kPassSpillFrom ? 1u : 0u;
-    P.spill_count = cls.spill_count;
-    P.spill_list = cls.spill_list;
+    P.order = io.in_list ? io.in_list : cls.order;
+    P.spill = io.out_list ? 1u : 0u;
+    P.spill_queue = io.out_queue;
+    P.spill_list = io.out_list;
+    P.narrow_bits = small ? io.narrow_bits : 0;
     P.length = db.length;
     P.sync = db.sync;
     P.expanded = db.expanded;
@@ -250,7 +286,7 @@ int ssw_context::launch_pass(const DeviceBatch &db, const ClassSpec &cls, const 
     P.witness_stride = db.witness_stride;
     P.want_witness = want_wit ? 1u : 0u;
     uint8_t *c = (uint8_t *)ctrs.p;
-    P.queues = cls.queues;
+    P.queues = io.in_queue ? io.in_queue : cls.queues;
     hipStream_t st = cls.stream ? cls.stream : stream;
     P.deferred_count = (unsigned int *)(c + kCtrDeferred);
     P.violation = (unsigned long long *)(c + kCtrViolation);
@@ -258,7 +294,7 @@ int ssw_context::launch_pass(const DeviceBatch &db, const ClassSpec &cls, const 
     P.deferred_list = (uint32_t *)deferred.p;
     P.chunk = (uint32_t)std::max(1, t.chunk);
     P.lds_n = use_lds ? nmax : 0;
-    const uint32_t bm_words = small ? SmallLds::bitmap_bytes(nmax) / 4u
+    const uint32_t bm_words = small ? SmallLds::bitmap_bytes(P.narrow_bits > 0 ? P.narrow_bits : nmax) / 4u
                                     : (use_lds ? (uint32_t)align_up(nmax >= 5 ? (1u << (nmax - 5)) : 1u, 4) : 4u);
     P.lds_bitmap_words = bm_words;
     const uint64_t all_subsets = 1ull << nmax;
@@ -267,7 +303,7 @@ int ssw_context::launch_pass(const DeviceBatch &db, const ClassSpec &cls, const 
         // a ring holding every subset of Q never has to leave LDS; a spilling
         // pass only needs room for one narrow step; groups of several
         // wavefronts check whole levels against the ring instead of steps
-        const int min_ring = G > 64 ? 256 : (pass == kPassSpillFrom ? 4 * G : 4 * G * T);
+        const int min_ring = G > 64 ? 256 : (io.out_list ? 4 * G : 4 * G * T);
         arena_lds = next_pow2((uint64_t)std::max(t.arena_lds, min_ring));
         arena_lds = std::min(arena_lds, next_pow2(all_subsets + 2ull * G * T));
     } else {
@@ -350,7 +386,8 @@ int ssw_context::launch_pass(const DeviceBatch &db, const ClassSpec &cls, const 
         while ((uint64_t)grid * groups_per_block > max_slots && grid > 1) grid = grid / 2;
     }
     const uint64_t slots = (uint64_t)grid * groups_per_block;
-    SlotKey key{G, block, P.lds_n, nmax, (int)want_wit, (int)arena_lds, (plain ? 1 : 0) + 2 * variant + 8 * pass};
+    SlotKey key{G, block, P.lds_n, nmax, (int)want_wit, (int)arena_lds,
+                (plain ? 1 : 0) + 2 * variant + 32 * io.kind + 128 * io.narrow_bits};
     DevBuf *sb = nullptr;
     int rc = get_slots(key, slots * P.slot_bytes, &sb, st);
     if (rc) return rc;
@@ -361,7 +398,7 @@ int ssw_context::launch_pass(const DeviceBatch &db, const ClassSpec &cls, const 
     last_variant = variant;
 
     if (!plain) {
-        HIP_TRY(init_queues(P.queues, P.count, st, pass == kPassSpillTo ? cls.spill_count : nullptr), "init queues");
+        if (!io.in_queue) HIP_TRY(init_queues(P.queues, P.count, st, false), "init queues");
         HIP_TRY(launch_batch(variant, G, P, grid, block, lds, st), "batch kernel launch");
         ++launches;
     } else {
@@ -458,14 +495,12 @@ int ssw_context::solve_device(const DeviceBatch &db, const uint8_t *host_n, bool
         } else {
             class_start.push_back(0);
         }
-        rc = ensure(queues, kQueueBytes * classes.size(), "work queues");
+        rc = ensure(queues, kQueueBytes * kClassQueues * classes.size(), "work queues");
         if (rc) return rc;
-        // ring-spill lists: a counter per class (own 256-byte line), then one
-        // list slot per automaton at the automaton's position in class order
-        const size_t spill_ctr_bytes = kSpillCtrBytes * classes.size();
-        rc = ensure(spill, spill_ctr_bytes + 4ull * db.count, "spill lists");
+        // spill lists: two per class, one slot per automaton at its position
+        // in class order
+        rc = ensure(spill, 8ull * db.count, "spill lists");
         if (rc) return rc;
-        HIP_TRY(fill_u32((uint32_t *)spill.p, spill_ctr_bytes / 4, 0, stream), "fill");
         const int nstreams = std::min<int>((int)classes.size(), kClassStreams);
         HIP_TRY(hipEventRecord(ev_fork, stream), "hipEventRecord");
         for (int k = 0; k < nstreams; ++k) HIP_TRY(hipStreamWaitEvent(class_streams[k], ev_fork, 0), "wait fork");
@@ -480,9 +515,11 @@ int ssw_context::solve_device(const DeviceBatch &db, const uint8_t *host_n, bool
             cls.host_n = nullptr;
             cls.plain_n = 0;
             cls.stream = nstreams > 1 ? class_streams[k % nstreams] : stream;
-            cls.queues = (uint32_t *)((uint8_t *)queues.p + kQueueBytes * k);
-            cls.spill_count = (uint32_t *)((uint8_t *)spill.p + kSpillCtrBytes * k);
-            cls.spill_list = (uint32_t *)((uint8_t *)spill.p + spill_ctr_bytes) + class_start[k];
+            cls.queues = (uint32_t *)((uint8_t *)queues.p + kQueueBytes * kClassQueues * k);
+            for (int j = 0; j < 2; ++j) {
+                cls.spill_queue[j] = (uint32_t *)((uint8_t *)cls.queues + kQueueBytes * (j + 1));
+                cls.spill_list[j] = (uint32_t *)spill.p + j * db.count + class_start[k];
+            }
             rc = launch_class(db, cls, false);
             if (rc) return rc;
             ++k;
@@ -827,6 +864,8 @@ int ssw_set_tuning(ssw_context *ctx, int32_t n, const ssw_tuning *t) {
     const int SG = t->spill_group_lanes;
     if (SG != 0 && SG != 16 && SG != 32 && SG != 64 && SG != 128 && SG != 256)
         return ctx->fail(SSW_ERR_ARG, "spill_group_lanes %d", SG);
+    if (t->narrow_bits != 0 && (t->narrow_bits < 8 || t->narrow_bits > 16))
+        return ctx->fail(SSW_ERR_ARG, "narrow_bits %d", t->narrow_bits);
     ctx->tune[n] = *t;
     return SSW_OK;
 }

@@ -43,6 +43,7 @@ bool valid_config(const ssw_tuning &t) {
     if (G == 256 && t.block_threads && t.block_threads != 256) return false;
     const int SG = t.spill_group_lanes;
     if (SG != 0 && SG != 16 && SG != 32 && SG != 64 && SG != 128 && SG != 256) return false;
+    if (t.narrow_bits != 0 && (t.narrow_bits < 8 || t.narrow_bits > 16)) return false;
     return t.chunk >= 1 && t.arena_lds >= 4 && t.spill_arena_lds >= 0 && t.spill_lane_entries >= 0;
 }
 
@@ -111,6 +112,13 @@ int tune_one(ssw_context &ctx, int n, const DeviceBatch &db, Timed *out) {
             if (best.t.spill_group_lanes > 64 &&
                 (rc = sweep(ctx, n, db, best, {1, 2, 4}, [](ssw_tuning &t, int v) { t.spill_lane_entries = v; })))
                 return rc;
+            if (n > 8 && (rc = sweep(ctx, n, db, best, {0, n - 1, n - 2}, [](ssw_tuning &t, int v) {
+                              t.narrow_bits = v >= 8 ? v : 0;
+                          })))
+                return rc;
+            if (best.t.narrow_bits &&
+                (rc = sweep(ctx, n, db, best, {0, 64, 128, 192, 256}, [](ssw_tuning &t, int v) { t.block_threads = v; })))
+                return rc;
         }
     }
     *out = best;
@@ -127,9 +135,9 @@ void write_json(FILE *f, ssw_context &ctx, int n_lo, int n_hi, int64_t samples, 
         fprintf(f,
                 "    \"%d\": {\"group_lanes\": %d, \"block_threads\": %d, \"chunk\": %d, \"arena_lds\": %d, "
                 "\"lane_entries\": %d, \"spill_arena_lds\": %d, \"spill_lane_entries\": %d, "
-                "\"spill_group_lanes\": %d, \"ms\": %.4f}%s\n",
+                "\"spill_group_lanes\": %d, \"narrow_bits\": %d, \"ms\": %.4f}%s\n",
                 n, t.group_lanes, t.block_threads, t.chunk, t.arena_lds, t.lane_entries, t.spill_arena_lds,
-                t.spill_lane_entries, t.spill_group_lanes, ms[n - n_lo], n < n_hi ? "," : "");
+                t.spill_lane_entries, t.spill_group_lanes, t.narrow_bits, ms[n - n_lo], n < n_hi ? "," : "");
     }
     fprintf(f, "  }\n}\n");
 }
@@ -242,6 +250,7 @@ int ssw_load_tuning(ssw_context *ctx, const char *path) {
         read_int(obj, "spill_arena_lds", &t.spill_arena_lds);
         read_int(obj, "spill_lane_entries", &t.spill_lane_entries);
         read_int(obj, "spill_group_lanes", &t.spill_group_lanes);
+        read_int(obj, "narrow_bits", &t.narrow_bits);
         if (n >= 1 && n <= SSW_MAX_N && valid_config(t)) {
             ctx->tune[n] = t;
             ++loaded;

@@ -61,7 +61,7 @@ class Tuning(ctypes.Structure):
     _fields_ = [("group_lanes", ctypes.c_int32), ("block_threads", ctypes.c_int32),
                 ("chunk", ctypes.c_int32), ("arena_lds", ctypes.c_int32), ("lane_entries", ctypes.c_int32),
                 ("spill_arena_lds", ctypes.c_int32), ("spill_lane_entries", ctypes.c_int32),
-                ("spill_group_lanes", ctypes.c_int32)]
+                ("spill_group_lanes", ctypes.c_int32), ("narrow_bits", ctypes.c_int32)]
 
 
 class Progress(ctypes.Structure):
@@ -359,12 +359,13 @@ class Context:
         self._check(self._L.ssw_get_tuning(self.h, n, ctypes.byref(t)), "get_tuning")
         return dict(group_lanes=t.group_lanes, block_threads=t.block_threads, chunk=t.chunk,
                     arena_lds=t.arena_lds, lane_entries=t.lane_entries, spill_arena_lds=t.spill_arena_lds,
-                    spill_lane_entries=t.spill_lane_entries, spill_group_lanes=t.spill_group_lanes)
+                    spill_lane_entries=t.spill_lane_entries, spill_group_lanes=t.spill_group_lanes,
+                    narrow_bits=t.narrow_bits)
 
     def set_tuning(self, n, group_lanes, block_threads, chunk, arena_lds, lane_entries=1, spill_arena_lds=0,
-                   spill_lane_entries=0, spill_group_lanes=0):
+                   spill_lane_entries=0, spill_group_lanes=0, narrow_bits=0):
         t = Tuning(group_lanes, block_threads, chunk, arena_lds, lane_entries, spill_arena_lds, spill_lane_entries,
-                   spill_group_lanes)
+                   spill_group_lanes, narrow_bits)
         self._check(self._L.ssw_set_tuning(self.h, n, ctypes.byref(t)), "set_tuning")
 
     def autotune(self, n_lo, n_hi, samples=20000, seed=7, out_path=None):

@@ -152,6 +152,10 @@ typedef struct ssw_tuning {
     int32_t spill_lane_entries; /* frontier entries per lane in that second launch: 1, 2 or 4 */
     int32_t spill_group_lanes;  /* threads per automaton in that second launch: 0 (= group_lanes),
                                    16, 32, 64, 128 or 256 */
+    int32_t narrow_bits;        /* n <= 16 with spill on: 0, or 8..n-1 = the first launch relabels the
+                                   states so that a(Q) | b(Q) takes the lowest labels and uses a
+                                   2^narrow_bits-bit visited bitmap; automata with a wider a(Q) | b(Q)
+                                   go through an extra launch with the full bitmap */
 } ssw_tuning;
 
 int ssw_get_tuning(ssw_context *ctx, int32_t n, ssw_tuning *out);

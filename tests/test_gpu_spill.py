@@ -47,7 +47,7 @@ def case():
 
 
 # (group_lanes, block_threads, chunk, arena_lds, lane_entries, spill_arena_lds, spill_lane_entries,
-#  spill_group_lanes)
+#  spill_group_lanes[, narrow_bits])
 CONFIGS = [
     (64, 128, 16, 512, 1, 4096, 4, 0),      # the committed n = 16 configuration
     (64, 0, 4, 256, 1, 4096, 2, 0),
@@ -59,6 +59,10 @@ CONFIGS = [
     (256, 256, 2, 512, 2, 4096, 4, 0),      # four wavefronts per automaton, spilling from the first launch
     (256, 256, 2, 1024, 4, 0, 0, 0),        # no spill: level-split groups with HBM relocation
     (64, 0, 2, 2048, 4, 0, 0, 0),           # no spill: one wavefront, frontier moves to HBM
+    (64, 128, 16, 512, 1, 4096, 4, 0, 15),  # relabelled states, 2^15-bit bitmap, three launches
+    (64, 128, 16, 512, 1, 4096, 4, 0, 13),
+    (32, 0, 8, 512, 1, 4096, 4, 0, 14),
+    (256, 256, 2, 1024, 2, 4096, 4, 0, 14), # relabelled, four wavefronts per automaton
 ]
 
 
@@ -79,8 +83,9 @@ def test_configuration_parity(ctx, case, cfg):
 
 
 def test_spill_mixed_sizes(ctx):
-    """Uneven sizes with spill on for n = 12..16: every class spills into its
-    own list and second launch, concurrently with the others."""
+    """Uneven sizes with spill on for n = 12..16 (relabelled states with an
+    (n-1)-bit bitmap for odd n): every class spills into its own lists and
+    later launches, concurrently with the others."""
     rng = np.random.default_rng(23)
     ns = rng.integers(12, 17, size=4000).astype(np.uint8)
     As, Bs = [], []
@@ -95,7 +100,7 @@ def test_spill_mixed_sizes(ctx):
     saved = {n: ctx.get_tuning(n) for n in range(12, 17)}
     try:
         for n in range(12, 17):
-            ctx.set_tuning(n, 64, 0, 8, 256, 1, 4096, 4, 0)
+            ctx.set_tuning(n, 64, 0, 8, 256, 1, 4096, 4, 0, n - 1 if n % 2 else 0)
         r = ctx.solve_mixed(ns, As, Bs)
     finally:
         for n, t in saved.items():
