@@ -761,6 +761,8 @@ struct SmallSolver {
     uint32_t rmask, rcap;
     bool in_lds;
     bool narrow;  // relabelled states, visited bitmap of 2^narrow_bits bits
+    uint32_t root_a, root_b;  // n > 16: a(Q), b(Q) in the relabelled states
+    uint32_t e0;              // subsets counted outside the ring (Q when n > 16)
     int64_t ai;
     int32_t n;
     uint32_t Q;
@@ -876,7 +878,8 @@ struct SmallSolver {
         n = uni<G>(nx_n);
         const uint32_t r = g.rank();
         const uint32_t b0 = nx_b0, b1 = nx_b1;
-        const bool bad_n = n < 1 || n > P.lds_n;
+        // n > 16 only through the relabelled path (a wavefront stages 2n <= 60 bytes)
+        const bool bad_n = n < 1 || n > P.lds_n || (n > 16 && (!narrow || G < 64));
         const uint32_t two_n = bad_n ? 0u : 2u * (uint32_t)n;
         bool badd = (r < two_n && b0 >= (uint32_t)n) || (G < 32 && r + G < two_n && b1 >= (uint32_t)n);
         if (r < two_n) *l8(stage_a + r) = (uint8_t)b0;
@@ -901,9 +904,13 @@ struct SmallSolver {
             // Q).  The states are relabelled so that U takes the |U| lowest
             // labels; the visited bitmap then needs 2^|U| bits, and automata
             // with |U| above the bitmap's width go to the next launch.
+            // Automata with n > 16 (lanes 0..n-1, n <= 30, one wavefront) are
+            // solved this way too when |U| <= 16; their root Q does not fit
+            // the 16-bit ring, so its level is expanded here.
+            constexpr int kTop = G >= 32 ? 16 : 8;
             uint32_t m = r < (uint32_t)n ? (1u << qa) | (1u << qb) : 0u;
 #pragma unroll
-            for (int o = 8; o >= 1; o >>= 1) m |= (uint32_t)__shfl_xor((int)m, o, 64);
+            for (int o = kTop; o >= 1; o >>= 1) m |= (uint32_t)__shfl_xor((int)m, o, 64);
             const uint32_t U = g.bcast0(m);
             const uint32_t k = (uint32_t)__popc(U);
             if (k > (uint32_t)P.narrow_bits) return spill();
@@ -911,11 +918,23 @@ struct SmallSolver {
                 const uint32_t below = (uint32_t)__popc(U & ((1u << x) - 1u));
                 return ((U >> x) & 1u) ? below : k + x - below;
             };
-            // singleton images of the relabelled states, zero past n
-            if (r < (uint32_t)n)
-                *l32(pimg_a + 4u * label(r)) = (1u << label(qa)) | (1u << (16u + label(qb)));
-            else if (r < 16u)
+            // singleton images of the relabelled states 0..15 (all of U), zero past n
+            if (r < (uint32_t)n) {
+                const uint32_t lr = label(r);
+                if (lr < 16u) *l32(pimg_a + 4u * lr) = (1u << label(qa)) | (1u << (16u + label(qb)));
+            } else if (r < 16u) {
                 *l32(pimg_a + 4u * r) = 0u;
+            }
+            if (n > 16) {
+                uint32_t ra = r < (uint32_t)n ? 1u << label(qa) : 0u, rb = r < (uint32_t)n ? 1u << label(qb) : 0u;
+#pragma unroll
+                for (int o = kTop; o >= 1; o >>= 1) {
+                    ra |= (uint32_t)__shfl_xor((int)ra, o, 64);
+                    rb |= (uint32_t)__shfl_xor((int)rb, o, 64);
+                }
+                root_a = g.bcast0(ra);
+                root_b = g.bcast0(rb);
+            }
             words = 1u << (P.narrow_bits - 5);
         } else {
             // singleton images of the 16 states, zero past n
@@ -949,20 +968,40 @@ struct SmallSolver {
             }
         }
         lsync();
+        // n > 16: level 1 = {a(Q), b(Q)} seeds the ring (Q counts once in E)
+        const bool pre = n > 16;
         if (r == 0) {
             // relabelled, Q lies outside the narrow bitmap and is never reached again
             if (!narrow) or_wg(l32(bm_a | ((Q >> 3) & 0x1FFCu)), 1u << (Q & 31u));
-            *l16(ring_a) = (uint16_t)Q;
+            if (pre) {
+                or_wg(l32(bm_a | ((root_a >> 3) & 0x1FFCu)), 1u << (root_a & 31u));
+                or_wg(l32(bm_a | ((root_b >> 3) & 0x1FFCu)), 1u << (root_b & 31u));
+                *l16(ring_a) = (uint16_t)root_a;
+                *l16(ring_a + 2u) = (uint16_t)root_b;
+            } else {
+                *l16(ring_a) = (uint16_t)Q;
+            }
             if constexpr (G > 64) {
 #pragma unroll
                 for (uint32_t k = 0; k < 6u; ++k) *l32(ctl_a + 4u * k) = 0u;
             }
         }
         lsync();
-        hi = 1;
-        wr = 1;
         rd = 0;
         level = 0;
+        e0 = 0;
+        if (pre) {
+            e0 = 1;
+            if (is_single(root_a) || is_single(root_b)) {
+                hi = wr = 0;
+                return kFound;  // length 1, one subset expanded
+            }
+            hi = wr = root_a == root_b ? 1u : 2u;
+            level = 1;
+            return kContinue;
+        }
+        hi = 1;
+        wr = 1;
         return n == 1 ? (int)kNoSync : (int)kContinue;
     }
 
@@ -1210,7 +1249,7 @@ struct SmallSolver {
         }
         const bool found = code == kFound;
         const int32_t L = found ? (int32_t)level + 1 : -1;
-        const int64_t E = hi;
+        const int64_t E = (int64_t)hi + e0;
         if (g.rank() == 0) {
             P.length[ai] = L;
             if (P.sync) P.sync[ai] = found ? 1 : 0;

@@ -72,6 +72,7 @@ struct PassIO {
     uint32_t *out_queue = nullptr;
     int narrow_bits = 0;                // relabelled states, 2^narrow_bits-bit bitmap
     bool byte_tab = false;
+    bool force_small = false;           // the n <= 16 solver on a class of larger n (relabelled)
 };
 constexpr int kClassQueues = 3;  // work queues per launch class: its own and two spill queues
 
@@ -154,7 +155,8 @@ struct ssw_context {
     hipStream_t stream = nullptr;
     hipStream_t monitor = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    static constexpr int kClassStreams = 8;   // concurrent launch classes of one call
+    static constexpr int kClassStreams = 8;        // streams for the launch classes of one call
+    static constexpr int kConcurrentClasses = 4;   // of which one call uses at most this many
     hipStream_t class_streams[kClassStreams] = {};
     hipEvent_t ev_fork = nullptr;
     hipEvent_t ev_join[kClassStreams] = {};

@@ -11,6 +11,7 @@
 #include "ssw_host.h"
 
 #include <algorithm>
+#include <cmath>
 #include <atomic>
 #include <chrono>
 #include <cstdio>
@@ -211,6 +212,31 @@ static inline uint32_t next_pow2(uint64_t x) {
 int ssw_context::launch_class(const DeviceBatch &db, const ClassSpec &cls, bool plain) {
     const ssw_tuning t = plain ? ssw_tuning{256, 256, 1, 8192, 1, 0, 0, 0, 0} : tune[cls.nmax];
     const bool small = cls.lds && db.witness == nullptr && cls.nmax <= kMaxLdsN;
+    if (!plain && !small && db.witness == nullptr && cls.nmax <= SSW_MAX_N && t.narrow_bits == 16 &&
+        cls.spill_queue[0]) {
+        // n > 16: automata with |a(Q) | b(Q)| <= 16 run relabelled in the
+        // n <= 16 solver (LDS bitmap, 16-bit ring; the first launch
+        // configuration of n = 16); the others, and frontiers that outgrow its
+        // ring, go on to this class's solver from a spill queue
+        hipStream_t st = cls.stream ? cls.stream : stream;
+        HIP_TRY(init_queues(cls.spill_queue[0], cls.items, st, true), "init queues");
+        ssw_tuning tp = tune[kMaxLdsN];
+        tp.group_lanes = 64;
+        tp.lane_entries = 1;
+        PassIO pre;
+        pre.kind = kPassSpillFrom;
+        pre.out_list = cls.spill_list[0];
+        pre.out_queue = cls.spill_queue[0];
+        pre.narrow_bits = 16;
+        pre.force_small = true;
+        int rc = launch_pass(db, cls, tp, false, pre);
+        if (rc) return rc;
+        PassIO rest;
+        rest.kind = kPassSpillTo;
+        rest.in_list = cls.spill_list[0];
+        rest.in_queue = cls.spill_queue[0];
+        return launch_pass(db, cls, t, false, rest);
+    }
     if (plain || !small || t.spill_arena_lds <= 0 || !cls.spill_queue[0])
         return launch_pass(db, cls, t, plain, PassIO{});
     const int nb = (t.narrow_bits >= 8 && t.narrow_bits < cls.nmax) ? t.narrow_bits : 0;
@@ -255,11 +281,11 @@ int ssw_context::launch_pass(const DeviceBatch &db, const ClassSpec &cls, const 
                              const PassIO &io) {
     const int G = t.group_lanes;
     int block = G > 64 ? G : std::max(64, std::min(256, t.block_threads ? t.block_threads : 256));
-    const bool use_lds = cls.lds;
+    const bool use_lds = cls.lds || io.force_small;
     const int nmax = cls.nmax;
     const bool want_wit = db.witness != nullptr;
     // the small path: bitmap in LDS, no witness, 16-bit ring frontier
-    const bool small = use_lds && !want_wit && nmax <= kMaxLdsN;
+    const bool small = io.force_small || (use_lds && !want_wit && nmax <= kMaxLdsN);
     const int T = t.lane_entries >= 4 ? 4 : (t.lane_entries >= 2 ? 2 : 1);
     // the last launch of a spill chain takes the wide searches: byte-sliced
     // image tables (two lookups per subset instead of four)
@@ -501,11 +527,34 @@ int ssw_context::solve_device(const DeviceBatch &db, const uint8_t *host_n, bool
         // in class order
         rc = ensure(spill, 8ull * db.count, "spill lists");
         if (rc) return rc;
-        const int nstreams = std::min<int>((int)classes.size(), kClassStreams);
+        // Streams: the classes are enqueued longest first, each on the stream
+        // with the least estimated work so far (at most kConcurrentClasses
+        // streams: the process has few hardware queues, and a long class
+        // queued behind short ones on a shared queue would start late).
+        // Estimated work: automata x 1.45^(n - 12), doubled for HBM bitmaps.
+        std::vector<std::pair<double, int>> by_cost;
+        {
+            int k = 0;
+            for (auto &kv : classes) {
+                double c = 0;
+                for (int n : kv.second) c += (double)hist[n] * std::pow(1.45, n - 12);
+                if (std::get<1>(kv.first) == 0) c *= 2.0;
+                by_cost.push_back({c, k++});
+            }
+            std::sort(by_cost.begin(), by_cost.end(), [](const std::pair<double, int> &x, const std::pair<double, int> &y) {
+                return x.first > y.first;
+            });
+        }
+        const int nstreams = std::min<int>((int)classes.size(), kConcurrentClasses);
+        std::vector<double> load((size_t)nstreams, 0.0);
         HIP_TRY(hipEventRecord(ev_fork, stream), "hipEventRecord");
         for (int k = 0; k < nstreams; ++k) HIP_TRY(hipStreamWaitEvent(class_streams[k], ev_fork, 0), "wait fork");
-        int k = 0;
-        for (auto &kv : classes) {
+        std::vector<std::pair<std::tuple<int, int, int>, std::vector<int>>> cl(classes.begin(), classes.end());
+        for (const auto &bc : by_cost) {
+            const int k = bc.second;
+            const auto &kv = cl[(size_t)k];
+            const int s = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            load[(size_t)s] += bc.first;
             ClassSpec cls;
             cls.lds = std::get<1>(kv.first) != 0;
             cls.nmax = *std::max_element(kv.second.begin(), kv.second.end());
@@ -514,7 +563,7 @@ int ssw_context::solve_device(const DeviceBatch &db, const uint8_t *host_n, bool
             cls.order = single_class ? nullptr : (const uint32_t *)order.p + class_start[k];
             cls.host_n = nullptr;
             cls.plain_n = 0;
-            cls.stream = nstreams > 1 ? class_streams[k % nstreams] : stream;
+            cls.stream = nstreams > 1 ? class_streams[s] : stream;
             cls.queues = (uint32_t *)((uint8_t *)queues.p + kQueueBytes * kClassQueues * k);
             for (int j = 0; j < 2; ++j) {
                 cls.spill_queue[j] = (uint32_t *)((uint8_t *)cls.queues + kQueueBytes * (j + 1));
@@ -522,7 +571,6 @@ int ssw_context::solve_device(const DeviceBatch &db, const uint8_t *host_n, bool
             }
             rc = launch_class(db, cls, false);
             if (rc) return rc;
-            ++k;
         }
         if (nstreams > 1)
             for (int j = 0; j < nstreams; ++j) {
@@ -864,8 +912,10 @@ int ssw_set_tuning(ssw_context *ctx, int32_t n, const ssw_tuning *t) {
     const int SG = t->spill_group_lanes;
     if (SG != 0 && SG != 16 && SG != 32 && SG != 64 && SG != 128 && SG != 256)
         return ctx->fail(SSW_ERR_ARG, "spill_group_lanes %d", SG);
-    if (t->narrow_bits != 0 && (t->narrow_bits < 8 || t->narrow_bits > 16))
-        return ctx->fail(SSW_ERR_ARG, "narrow_bits %d", t->narrow_bits);
+    if (t->narrow_bits != 0 && (t->narrow_bits < 8 || t->narrow_bits > 16 || (n <= 16 && t->narrow_bits >= n) ||
+                                (n > 16 && t->narrow_bits != 16)))
+        return ctx->fail(SSW_ERR_ARG, "narrow_bits %d for n=%d (8..n-1 for n <= 16, 16 for n > 16)",
+                         t->narrow_bits, n);
     ctx->tune[n] = *t;
     return SSW_OK;
 }

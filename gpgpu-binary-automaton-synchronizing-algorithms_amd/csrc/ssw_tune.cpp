@@ -86,6 +86,10 @@ int tune_one(ssw_context &ctx, int n, const DeviceBatch &db, Timed *out) {
     if ((rc = sweep(ctx, n, db, best, tiles, [](ssw_tuning &t, int v) { t.arena_lds = v; }))) return rc;
     if (small && (rc = sweep(ctx, n, db, best, {1, 2, 4}, [](ssw_tuning &t, int v) { t.lane_entries = v; })))
         return rc;
+    // n > 16: automata with |a(Q) | b(Q)| <= 16 first through the relabelled
+    // n <= 16 solver
+    if (!small && (rc = sweep(ctx, n, db, best, {0, 16}, [](ssw_tuning &t, int v) { t.narrow_bits = v; })))
+        return rc;
     if (small) {
         // ring spill: automata that outgrow a small first-launch ring go to a
         // second launch with a large ring; then the first ring can shrink

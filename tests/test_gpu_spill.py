@@ -112,3 +112,43 @@ def test_spill_mixed_sizes(ctx):
         L, S, E, _ = O.solve(n, A, B, threads=8)
         np.testing.assert_array_equal(r.length[idx], L, err_msg=f"n={n}")
         np.testing.assert_array_equal(r.expanded[idx], E, err_msg=f"n={n}")
+
+
+def test_relabelled_large_n(ctx):
+    """n = 17..24 through the relabelled n <= 16 solver: automata with
+    |a(Q) | b(Q)| <= 16 are solved there (root level expanded at start), the
+    others (Cerny C_n, wide random ones) by the HBM-bitmap solver from the
+    spill queue.  Checked against the oracle's uint32 search."""
+    rng = np.random.default_rng(31)
+    ns = rng.integers(17, 25, size=1500).astype(np.uint8)
+    As, Bs = [], []
+    for i, n in enumerate(ns):
+        if i % 151 == 0:
+            a, b = ssw.generate_cerny(int(n), i % 2)
+        elif i % 7 == 0:
+            # image of Q under a and b inside 12 states: |a(Q) | b(Q)| <= 12
+            a = rng.integers(0, 12, size=n)
+            b = rng.integers(0, 12, size=n)
+        else:
+            a = rng.integers(0, n, size=n)
+            b = rng.integers(0, n, size=n)
+        As.append(np.asarray(a, np.int32))
+        Bs.append(np.asarray(b, np.int32))
+    saved = {n: ctx.get_tuning(n) for n in range(16, 25)}
+    try:
+        for n in range(17, 25):
+            t = dict(saved[n])
+            t["narrow_bits"] = 16
+            ctx.set_tuning(n, **t)
+        r = ctx.solve_mixed(ns, As, Bs)
+    finally:
+        for n, t in saved.items():
+            ctx.set_tuning(n, **t)
+    for n in range(17, 25):
+        idx = np.nonzero(ns == n)[0]
+        A = np.stack([As[i] for i in idx])
+        B = np.stack([Bs[i] for i in idx])
+        L, S, E, _ = O.solve(n, A, B, threads=8)
+        np.testing.assert_array_equal(r.length[idx], L, err_msg=f"n={n}")
+        np.testing.assert_array_equal(r.sync[idx], S, err_msg=f"n={n}")
+        np.testing.assert_array_equal(r.expanded[idx], E, err_msg=f"n={n}")
