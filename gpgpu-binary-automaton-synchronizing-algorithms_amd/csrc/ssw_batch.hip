@@ -1128,13 +1128,13 @@ struct SmallSolver {
             const uint32_t y = x & (x - 0x00010001u);
             single |= ((uint16_t)y == 0) | (y < 0x10000u);
             const uint32_t ba = 1u << (x & 31u), bb = 1u << ((x >> 16) & 31u);
-            uint32_t oa = ~0u, ob = ~0u;
+            uint32_t oa, ob;  // read only where v[t]
             if (v[t]) {
                 oa = or_wg(l32(bm_a | ((x >> 3) & 0x1FFCu)), ba);
                 ob = or_wg(l32(bm_a | ((x >> 19) & 0x1FFCu)), bb);
             }
-            nw[2 * t] = (oa & ba) == 0u;
-            nw[2 * t + 1] = (ob & bb) == 0u;
+            nw[2 * t] = v[t] && (oa & ba) == 0u;
+            nw[2 * t + 1] = v[t] && (ob & bb) == 0u;
         }
         // wave-local flag-major positions, then one reservation on the
         // level's counter
@@ -1202,7 +1202,7 @@ struct SmallSolver {
             const uint32_t y = x & (x - 0x00010001u);
             single |= ((uint16_t)y == 0) | (y < 0x10000u);
             const uint32_t ba = 1u << (x & 31u), bb = 1u << ((x >> 16) & 31u);
-            uint32_t oa = ~0u, ob = ~0u;
+            uint32_t oa, ob;  // read only where v[t]
             if (v[t]) {
                 // issued back to back; the second sees the first, so a == b
                 // is counted once
@@ -1210,8 +1210,8 @@ struct SmallSolver {
                 oa = or_wg(l32(bm_a | ((x >> 3) & 0x1FFCu)), ba);
                 ob = or_wg(l32(bm_a | ((x >> 19) & 0x1FFCu)), bb);
             }
-            nw[2 * t] = (oa & ba) == 0u;
-            nw[2 * t + 1] = (ob & bb) == 0u;
+            nw[2 * t] = v[t] && (oa & ba) == 0u;
+            nw[2 * t + 1] = v[t] && (ob & bb) == 0u;
         }
         bool any_single;
         uint32_t pos[2 * TT], total;

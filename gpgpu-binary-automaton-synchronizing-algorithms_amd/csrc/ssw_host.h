@@ -201,6 +201,14 @@ struct ssw_context {
                     const ssw::PassIO &io);
     int solve_device(const ssw::DeviceBatch &db, const uint8_t *host_n, bool plain, int64_t *violation_out);
     int solve_host_packed(const ssw_batch *b, int mode, int64_t *violation_out);
+    // host staging for the int32 entry points: pinned, grown on demand
+    void *h_stage = nullptr;
+    size_t h_stage_bytes = 0;
+    int ensure_stage(size_t bytes);
+    // set by compute_i32 for the call it makes: transitions already checked,
+    // lengths not wanted by the caller (Verify)
+    bool host_checked = false;
+    bool host_no_length = false;
     int solve_entry(const ssw_batch *b, int32_t memory, int mode, int64_t *violation_out);
     int compute_i32(int32_t n, int64_t count, const int32_t *A, const int32_t *B, int32_t *length_out,
                     uint8_t *sync_out, uint8_t *witness_out, int32_t witness_stride, int64_t *expanded_out,

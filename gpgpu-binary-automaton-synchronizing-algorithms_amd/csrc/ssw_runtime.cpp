@@ -133,6 +133,8 @@ void ssw_context::close() {
     }
     if (stream) hipStreamDestroy(stream);
     if (monitor) hipStreamDestroy(monitor);
+    if (h_stage) hipHostFree(h_stage);
+    h_stage = nullptr;
 }
 
 void ssw_context::default_tuning() {
@@ -606,18 +608,67 @@ int ssw_context::solve_device(const DeviceBatch &db, const uint8_t *host_n, bool
 // host-buffer front ends
 // ---------------------------------------------------------------------------
 
+// Runs fn(lo, hi) over [0, count) on up to 16 host threads (one when the
+// range is short).
+template <class F>
+static void parallel_ranges(int64_t count, int64_t per_thread, F &&fn) {
+    const int64_t hw = std::max<int64_t>(1, (int64_t)std::thread::hardware_concurrency());
+    const int threads = (int)std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(hw, 16), count / per_thread));
+    if (threads == 1) {
+        fn((int64_t)0, count);
+        return;
+    }
+    std::vector<std::thread> ts;
+    for (int t = 0; t < threads; ++t)
+        ts.emplace_back([&fn, count, t, threads] { fn(count * t / threads, count * (t + 1) / threads); });
+    for (auto &th : ts) th.join();
+}
+
+int ssw_context::ensure_stage(size_t bytes) {
+    if (h_stage && h_stage_bytes >= bytes) return SSW_OK;
+    if (h_stage) hipHostFree(h_stage);
+    h_stage = nullptr;
+    h_stage_bytes = 0;
+    const size_t want = std::max<size_t>(bytes, 1u << 20);
+    HIP_TRY(hipHostMalloc(&h_stage, want, hipHostMallocDefault), "hipHostMalloc");
+    h_stage_bytes = want;
+    return SSW_OK;
+}
+
 int ssw_context::solve_host_packed(const ssw_batch *b, int mode, int64_t *violation_out) {
     const int64_t count = b->count;
-    // extent of the packed transitions and validation
+    // extent of the packed transitions and validation (first bad automaton)
     uint64_t extent = 0;
-    for (int64_t i = 0; i < count; ++i) {
-        const int n = b->uniform_n > 0 ? b->uniform_n : b->n_per[i];
-        if (n < 1 || n > SSW_MAX_N) return fail(SSW_ERR_ARG, "automaton %lld: n=%d out of [1,30]", (long long)i, n);
-        const uint64_t o = b->offset ? b->offset[i] : (uint64_t)i * 2u * (uint64_t)n;
-        const uint8_t *d = b->delta + o;
-        for (int q = 0; q < 2 * n; ++q)
-            if (d[q] >= n) return fail(SSW_ERR_ARG, "automaton %lld: transition %d >= n", (long long)i, d[q]);
-        extent = std::max<uint64_t>(extent, o + 2u * (uint64_t)n);
+    if (host_checked && b->uniform_n > 0 && !b->offset) {
+        extent = (uint64_t)count * 2u * (uint64_t)b->uniform_n;
+    } else {
+        std::mutex m;
+        int64_t bad = -1;
+        parallel_ranges(count, 1 << 16, [&](int64_t lo, int64_t hi) {
+            uint64_t ext = 0;
+            int64_t first = -1;
+            for (int64_t i = lo; i < hi && first < 0; ++i) {
+                const int n = b->uniform_n > 0 ? b->uniform_n : b->n_per[i];
+                if (n < 1 || n > SSW_MAX_N) {
+                    first = i;
+                    break;
+                }
+                const uint64_t o = b->offset ? b->offset[i] : (uint64_t)i * 2u * (uint64_t)n;
+                const uint8_t *d = b->delta + o;
+                for (int q = 0; q < 2 * n; ++q)
+                    if (d[q] >= n) first = i;
+                ext = std::max<uint64_t>(ext, o + 2u * (uint64_t)n);
+            }
+            std::lock_guard<std::mutex> lk(m);
+            extent = std::max(extent, ext);
+            if (first >= 0 && (bad < 0 || first < bad)) bad = first;
+        });
+        if (bad >= 0) {
+            const int n = b->uniform_n > 0 ? b->uniform_n : b->n_per[bad];
+            if (n < 1 || n > SSW_MAX_N)
+                return fail(SSW_ERR_ARG, "automaton %lld: n=%d out of [1,30]", (long long)bad, n);
+            return fail(SSW_ERR_ARG, "automaton %lld: a transition >= n=%d", (long long)bad, n);
+        }
     }
     int rc;
     if ((rc = ensure(d_delta, extent, "delta"))) return rc;
@@ -656,7 +707,8 @@ int ssw_context::solve_host_packed(const ssw_batch *b, int mode, int64_t *violat
                             : solve_device(db, b->uniform_n > 0 ? nullptr : b->n_per, mode == kModePlain,
                                            violation_out);
     if (rc) return rc;
-    HIP_TRY(hipMemcpyAsync(b->length, db.length, 4ull * count, hipMemcpyDeviceToHost, stream), "D2H");
+    if (!host_no_length)
+        HIP_TRY(hipMemcpyAsync(b->length, db.length, 4ull * count, hipMemcpyDeviceToHost, stream), "D2H");
     if (b->sync) HIP_TRY(hipMemcpyAsync(b->sync, db.sync, count, hipMemcpyDeviceToHost, stream), "D2H");
     if (b->expanded)
         HIP_TRY(hipMemcpyAsync(b->expanded, db.expanded, 8ull * count, hipMemcpyDeviceToHost, stream), "D2H");
@@ -730,31 +782,58 @@ int ssw_context::compute_i32(int32_t n, int64_t count, const int32_t *A, const i
                              int32_t witness_stride, int64_t *expanded_out, int32_t bound,
                              int64_t *violation_out) {
     if (n < 1 || n > SSW_MAX_N) return fail(SSW_ERR_ARG, "n=%d out of [1,30]", n);
-    if (count < 0 || (count > 0 && (!A || !B || !length_out))) return fail(SSW_ERR_ARG, "null argument");
-    // flatten like SlimGPUQueue.cs:108-126, packed to bytes
-    std::vector<uint8_t> packed((size_t)count * 2 * n);
-    for (int64_t i = 0; i < count; ++i) {
-        uint8_t *d = packed.data() + (size_t)i * 2 * n;
-        for (int q = 0; q < n; ++q) {
-            const int32_t x = A[i * n + q], y = B[i * n + q];
-            if (x < 0 || x >= n || y < 0 || y >= n)
-                return fail(SSW_ERR_ARG, "automaton %lld state %d: transition out of [0,%d)", (long long)i, q, n);
-            d[q] = (uint8_t)x;
-            d[n + q] = (uint8_t)y;
+    if (count < 0 || (count > 0 && (!A || !B))) return fail(SSW_ERR_ARG, "null argument");
+    // flatten like SlimGPUQueue.cs:108-126, packed to bytes in pinned staging
+    // memory by up to 16 host threads, checked on the way
+    int rc = ensure_stage((size_t)count * 2 * n);
+    if (rc) return rc;
+    uint8_t *packed = (uint8_t *)h_stage;
+    std::mutex m;
+    int64_t bad = -1;
+    parallel_ranges(count, 1 << 15, [&](int64_t lo, int64_t hi) {
+        int64_t first = -1;
+        for (int64_t i = lo; i < hi; ++i) {
+            uint8_t *d = packed + (size_t)i * 2 * n;
+            bool ok = true;
+            for (int q = 0; q < n; ++q) {
+                const int32_t x = A[i * n + q], y = B[i * n + q];
+                ok &= (uint32_t)x < (uint32_t)n && (uint32_t)y < (uint32_t)n;
+                d[q] = (uint8_t)x;
+                d[n + q] = (uint8_t)y;
+            }
+            if (!ok) {
+                first = i;
+                break;
+            }
         }
+        if (first >= 0) {
+            std::lock_guard<std::mutex> lk(m);
+            if (bad < 0 || first < bad) bad = first;
+        }
+    });
+    if (bad >= 0) {
+        int q = 0;
+        while (q < n && (uint32_t)A[bad * n + q] < (uint32_t)n && (uint32_t)B[bad * n + q] < (uint32_t)n) ++q;
+        return fail(SSW_ERR_ARG, "automaton %lld state %d: transition out of [0,%d)", (long long)bad, q, n);
     }
+    static int32_t no_length;
     ssw_batch b;
     std::memset(&b, 0, sizeof b);
     b.count = count;
     b.uniform_n = n;
-    b.delta = packed.data();
-    b.length = length_out;
+    b.delta = packed;
+    b.length = length_out ? length_out : &no_length;
     b.sync = sync_out;
     b.expanded = expanded_out;
     b.witness = witness_out;
     b.witness_stride = witness_stride;
     b.bound = bound;
-    return solve_entry(&b, SSW_HOST, kModeBatch, violation_out);
+    host_checked = true;
+    host_no_length = length_out == nullptr;
+    rc = solve_entry(&b, SSW_HOST, kModeBatch, violation_out);
+    host_checked = false;
+    host_no_length = false;
+    return rc;
 }
 
 // ---------------------------------------------------------------------------
@@ -856,6 +935,7 @@ int ssw_compute(ssw_context *ctx, int32_t n, int64_t count, const int32_t *delta
                 const int32_t *delta_b, int32_t *length_out, uint8_t *sync_out, uint8_t *witness_out,
                 int32_t witness_stride, int64_t *expanded_out, int32_t bound, int64_t *violation_out) {
     CTX_GUARD(ctx);
+    if (count > 0 && !length_out) return ctx->fail(SSW_ERR_ARG, "null argument");
     return ctx->compute_i32(n, count, delta_a, delta_b, length_out, sync_out, witness_out, witness_stride,
                             expanded_out, bound, violation_out);
 }
@@ -863,9 +943,7 @@ int ssw_compute(ssw_context *ctx, int32_t n, int64_t count, const int32_t *delta
 int ssw_verify(ssw_context *ctx, int32_t n, int64_t count, const int32_t *delta_a, const int32_t *delta_b,
                int32_t bound, int64_t *violation_out) {
     CTX_GUARD(ctx);
-    std::vector<int32_t> len((size_t)std::max<int64_t>(count, 0));
-    return ctx->compute_i32(n, count, delta_a, delta_b, len.data(), nullptr, nullptr, 0, nullptr, bound,
-                            violation_out);
+    return ctx->compute_i32(n, count, delta_a, delta_b, nullptr, nullptr, nullptr, 0, nullptr, bound, violation_out);
 }
 
 int ssw_best_parallelism(ssw_context *ctx) { return ctx ? 1 : SSW_ERR_ARG; }

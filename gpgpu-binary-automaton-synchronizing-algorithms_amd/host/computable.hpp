@@ -132,9 +132,13 @@ public:
         const int stride = std::max(1, ((n - 1) * (n - 1) + 7) / 8);
         std::vector<uint8_t> words(want_words ? (size_t)problemCount * stride : 0);
         int64_t violation = -1;
-        const int rc = ssw_compute(ctx_.get(), n, problemCount, a.data(), b.data(), length.data(), nullptr,
-                                   want_words ? words.data() : nullptr, stride, expanded.data(), -1, &violation);
-        if (rc != SSW_OK) throw std::runtime_error(std::string("ssw_compute: ") + ssw_last_error(ctx_.get()));
+        // Verify (no results wanted): only the violating index comes back
+        const int rc = results ? ssw_compute(ctx_.get(), n, problemCount, a.data(), b.data(), length.data(), nullptr,
+                                             want_words ? words.data() : nullptr, stride, expanded.data(), -1,
+                                             &violation)
+                               : ssw_verify(ctx_.get(), n, problemCount, a.data(), b.data(), -1, &violation);
+        if (rc != SSW_OK)
+            throw std::runtime_error(std::string(results ? "ssw_compute: " : "ssw_verify: ") + ssw_last_error(ctx_.get()));
         const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         if (results) {
             for (int i = 0; i < problemCount; ++i) {
