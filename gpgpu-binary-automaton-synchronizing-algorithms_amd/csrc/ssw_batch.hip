@@ -135,14 +135,19 @@ struct WaveGroup {
     // any(single), and for every flag f[j] of every lane its position in a
     // flag-major layout starting at base: all set f[0] of the group in lane
     // order, then all set f[1], ...; total = number of set flags
+    // From wavefront ballots: sm = lanes with a singleton image, f[j] = lanes
+    // whose flag j is set.  any(single), and for every flag of every lane its
+    // position in a flag-major layout starting at base: all set flags 0 of
+    // the group in lane order, then all set flags 1, ...; total = number of
+    // set flags
     template <int K>
-    __device__ void place(bool single, const bool (&f)[K], uint32_t base, bool &any_single, uint32_t (&pos)[K],
+    __device__ void place(uint64_t sm, const uint64_t (&f)[K], uint32_t base, bool &any_single, uint32_t (&pos)[K],
                           uint32_t &total) const {
-        any_single = (__ballot(single) & gmask) != 0;
+        any_single = (sm & gmask) != 0;
         uint32_t b = base;
 #pragma unroll
         for (int j = 0; j < K; ++j) {
-            const uint64_t m = __ballot(f[j]) & gmask;
+            const uint64_t m = f[j] & gmask;
             pos[j] = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, b));
             b += (uint32_t)__popcll(m);
         }
@@ -269,13 +274,13 @@ struct BlockGroup {
         any_single = __builtin_amdgcn_readfirstlane(anys ? 1u : 0u) != 0u;
     }
     template <int K>
-    __device__ void place(bool single, const bool (&f)[K], uint32_t base, bool &any_single, uint32_t (&pos)[K],
+    __device__ void place(uint64_t sm, const uint64_t (&f)[K], uint32_t base, bool &any_single, uint32_t (&pos)[K],
                           uint32_t &total) const {
-        const uint64_t bs = __ballot(single);
+        const uint64_t bs = sm;
         uint32_t pre[K], cnt[K];
 #pragma unroll
         for (int j = 0; j < K; ++j) {
-            const uint64_t m = __ballot(f[j]);
+            const uint64_t m = f[j];
             pre[j] = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
             cnt[j] = (uint32_t)__popcll(m);
         }
@@ -1013,6 +1018,18 @@ struct SmallSolver {
 
     __device__ int step() {
         if constexpr (G > 64) return step_level();
+        if constexpr (T == 1 && G == 64) {
+            // one entry per lane, one wavefront, LDS ring: the steps of the
+            // search run in one tight loop with a single path (the launch that
+            // takes every automaton spends most of its instructions here);
+            // the general path below takes over when the ring is short
+            if (in_lds) {
+                while (wr + 2u * G - rd <= rcap) {
+                    const int c = step_in<true, 1>();
+                    if (c != kContinue) return c;
+                }
+            }
+        }
         // entries per lane: the fewest steps for the width left in the level
         // (a step's instruction count grows slower than its entries, so one
         // T = 4 step over 150..256 entries costs less than T = 2 plus T = 1
@@ -1120,30 +1137,16 @@ struct SmallSolver {
         uint32_t img[TT];
 #pragma unroll
         for (int t = 0; t < TT; ++t) img[t] = image(s[t]);
-        bool single = false;
         bool nw[2 * TT];
-#pragma unroll
-        for (int t = 0; t < TT; ++t) {
-            const uint32_t x = img[t];
-            const uint32_t y = x & (x - 0x00010001u);
-            single |= ((uint16_t)y == 0) | (y < 0x10000u);
-            const uint32_t ba = 1u << (x & 31u), bb = 1u << ((x >> 16) & 31u);
-            uint32_t oa, ob;  // read only where v[t]
-            if (v[t]) {
-                oa = or_wg(l32(bm_a | ((x >> 3) & 0x1FFCu)), ba);
-                ob = or_wg(l32(bm_a | ((x >> 19) & 0x1FFCu)), bb);
-            }
-            nw[2 * t] = v[t] && (oa & ba) == 0u;
-            nw[2 * t + 1] = v[t] && (ob & bb) == 0u;
-        }
+        uint64_t m[2 * TT], sm;
+        expand<TT>(img, v, nw, m, sm);
         // wave-local flag-major positions, then one reservation on the
         // level's counter
         uint32_t pos[2 * TT], total = 0;
 #pragma unroll
         for (int j = 0; j < 2 * TT; ++j) {
-            const uint64_t m = __ballot(nw[j]);
-            pos[j] = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, total));
-            total += (uint32_t)__popcll(m);
+            pos[j] = __builtin_amdgcn_mbcnt_hi((uint32_t)(m[j] >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m[j], total));
+            total += (uint32_t)__popcll(m[j]);
         }
         total = __builtin_amdgcn_readfirstlane(total);
         if (total) {
@@ -1161,7 +1164,7 @@ struct SmallSolver {
                 }
             }
         }
-        return __ballot(single) != 0;
+        return sm != 0;
     }
 
     // Hands the automaton to the second launch (larger LDS ring).
@@ -1171,6 +1174,42 @@ struct SmallSolver {
             P.spill_list[k] = (uint32_t)ai;
         }
         return kSpilled;
+    }
+
+    // Images img[t] of up to TT entries per lane (v[t]: entry in the level):
+    // their test-and-sets, nw = the image is new, and the wavefront ballots
+    // m[2t], m[2t+1] (new images under a, b) and sm (a singleton image).
+    // Every ballot is of a plain compare, so the lane masks are used as the
+    // compares produce them.  A lane past the level repeats entry rd (or the
+    // chunk's first entry): its singleton test is that of the lane owning that
+    // entry, and it skips the test-and-sets (its results read as not new).
+    // The halves of y = x & (x - 0x10001) are a & (a - 1) and b & (b - 1) of
+    // the nonempty images a (low half) and b (high half).  Q is never a
+    // singleton here (n = 1 ends at start), so a singleton image is always
+    // new; it is still marked and appended, which is harmless because the
+    // search stops at this level.
+    template <int TT>
+    __device__ __forceinline__ void expand(const uint32_t (&img)[TT], const bool (&v)[TT], bool (&nw)[2 * TT],
+                                           uint64_t (&m)[2 * TT], uint64_t &sm) const {
+        sm = 0;
+#pragma unroll
+        for (int t = 0; t < TT; ++t) {
+            const uint32_t x = img[t];
+            const uint32_t y = x & (x - 0x00010001u);
+            sm |= __ballot((y & 0xFFFFu) == 0u) | __ballot(y < 0x10000u);
+            const uint32_t ba = 1u << (x & 31u), bb = 1u << ((x >> 16) & 31u);
+            uint32_t oa = ~0u, ob = ~0u;
+            if (v[t]) {
+                // issued back to back; the second sees the first, so a == b
+                // is counted once; word of a: bits 5..15 of x, of b: bits 21..31
+                oa = or_wg(l32(bm_a | ((x >> 3) & 0x1FFCu)), ba);
+                ob = or_wg(l32(bm_a | ((x >> 19) & 0x1FFCu)), bb);
+            }
+            nw[2 * t] = (oa & ba) == 0u;
+            nw[2 * t + 1] = (ob & bb) == 0u;
+            m[2 * t] = __ballot(nw[2 * t]);
+            m[2 * t + 1] = __ballot(nw[2 * t + 1]);
+        }
     }
 
     template <bool LDS, int TT>
@@ -1189,33 +1228,12 @@ struct SmallSolver {
         uint32_t img[TT];
 #pragma unroll
         for (int t = 0; t < TT; ++t) img[t] = image(s[t]);
-        bool single = false;
         bool nw[2 * TT];
-#pragma unroll
-        for (int t = 0; t < TT; ++t) {
-            // a = low half, b = high half, both nonempty; y holds a & (a - 1)
-            // and b & (b - 1) in its halves.  Q is never a singleton here
-            // (n = 1 ends at start), so a singleton image is always new; it is
-            // still marked and appended, which is harmless because the search
-            // stops at this level.
-            const uint32_t x = img[t];
-            const uint32_t y = x & (x - 0x00010001u);
-            single |= ((uint16_t)y == 0) | (y < 0x10000u);
-            const uint32_t ba = 1u << (x & 31u), bb = 1u << ((x >> 16) & 31u);
-            uint32_t oa, ob;  // read only where v[t]
-            if (v[t]) {
-                // issued back to back; the second sees the first, so a == b
-                // is counted once
-                // word of a: bits 5..15 of x; word of b: bits 21..31
-                oa = or_wg(l32(bm_a | ((x >> 3) & 0x1FFCu)), ba);
-                ob = or_wg(l32(bm_a | ((x >> 19) & 0x1FFCu)), bb);
-            }
-            nw[2 * t] = v[t] && (oa & ba) == 0u;
-            nw[2 * t + 1] = v[t] && (ob & bb) == 0u;
-        }
+        uint64_t m[2 * TT], sm;
+        expand<TT>(img, v, nw, m, sm);
         bool any_single;
         uint32_t pos[2 * TT], total;
-        g.template place<2 * TT>(single, nw, wr, any_single, pos, total);
+        g.template place<2 * TT>(sm, m, wr, any_single, pos, total);
 #pragma unroll
         for (int j = 0; j < 2 * TT; ++j) {
             const uint32_t val = (j & 1) ? (img[j >> 1] >> 16) : (img[j >> 1] & 0xFFFFu);
