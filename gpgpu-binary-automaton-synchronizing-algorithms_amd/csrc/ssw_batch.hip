@@ -1029,6 +1029,22 @@ struct SmallSolver {
                     if (c != kContinue) return c;
                 }
             }
+        } else if constexpr (G == 64) {
+            // the same for T > 1 (the spill launch): loop while the ring holds
+            // the widest step
+            if (in_lds) {
+                while (wr + 2u * G * T - rd <= rcap) {
+                    const uint32_t left = hi - rd;
+                    int c;
+                    if (T >= 4 && left > 2u * G)
+                        c = step_in<true, (T >= 4 ? 4 : 1)>();
+                    else if (T >= 2 && left > G)
+                        c = step_in<true, (T >= 2 ? 2 : 1)>();
+                    else
+                        c = step_in<true, 1>();
+                    if (c != kContinue) return c;
+                }
+            }
         }
         // entries per lane: the fewest steps for the width left in the level
         // (a step's instruction count grows slower than its entries, so one
