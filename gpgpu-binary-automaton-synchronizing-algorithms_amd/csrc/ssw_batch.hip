@@ -67,6 +67,17 @@ __device__ __forceinline__ int64_t uni(int64_t v) {
     return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 
+// Inclusive OR-scan within each row of 16 lanes (DPP row_shr 1, 2, 4, 8;
+// lanes shifted in from outside the row read 0): lane 16k+15 ends with the
+// OR of its row.
+__device__ __forceinline__ uint32_t row_or_scan(uint32_t m) {
+    m |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)m, 0x111, 0xF, 0xF, true);
+    m |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)m, 0x112, 0xF, 0xF, true);
+    m |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)m, 0x114, 0xF, 0xF, true);
+    m |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)m, 0x118, 0xF, 0xF, true);
+    return m;
+}
+
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();
@@ -159,6 +170,13 @@ struct WaveGroup {
     __device__ uint32_t bcast0(uint32_t v) const {
         if (G == 64) return __builtin_amdgcn_readfirstlane(v);
         return __shfl(v, (int)gbase, 64);
+    }
+    // OR of v over ranks 0..31 (0..G-1 when G < 32), for every lane of the group
+    __device__ uint32_t or_low32(uint32_t v) const {
+        v = row_or_scan(v);
+        if (G == 64) return __builtin_amdgcn_readlane(v, 15) | __builtin_amdgcn_readlane(v, 31);
+        if (G == 32) return (uint32_t)__shfl((int)v, (int)gbase + 15, 64) | (uint32_t)__shfl((int)v, (int)gbase + 31, 64);
+        return (uint32_t)__shfl((int)v, (int)gbase + 15, 64);
     }
 };
 
@@ -317,6 +335,11 @@ struct BlockGroup {
         if (tid == 0) scratch[2 * kWaves] = v;
         __syncthreads();
         return __builtin_amdgcn_readfirstlane(scratch[2 * kWaves]);
+    }
+    // OR of v over threads 0..31, for every thread
+    __device__ uint32_t or_low32(uint32_t v) const {
+        v = row_or_scan(v);
+        return bcast0(__builtin_amdgcn_readlane(v, 15) | __builtin_amdgcn_readlane(v, 31));
     }
 };
 
@@ -912,11 +935,7 @@ struct SmallSolver {
             // Automata with n > 16 (lanes 0..n-1, n <= 30, one wavefront) are
             // solved this way too when |U| <= 16; their root Q does not fit
             // the 16-bit ring, so its level is expanded here.
-            constexpr int kTop = G >= 32 ? 16 : 8;
-            uint32_t m = r < (uint32_t)n ? (1u << qa) | (1u << qb) : 0u;
-#pragma unroll
-            for (int o = kTop; o >= 1; o >>= 1) m |= (uint32_t)__shfl_xor((int)m, o, 64);
-            const uint32_t U = g.bcast0(m);
+            const uint32_t U = g.or_low32(r < (uint32_t)n ? (1u << qa) | (1u << qb) : 0u);
             const uint32_t k = (uint32_t)__popc(U);
             if (k > (uint32_t)P.narrow_bits) return spill();
             const auto label = [U, k](uint32_t x) {
@@ -931,14 +950,8 @@ struct SmallSolver {
                 *l32(pimg_a + 4u * r) = 0u;
             }
             if (n > 16) {
-                uint32_t ra = r < (uint32_t)n ? 1u << label(qa) : 0u, rb = r < (uint32_t)n ? 1u << label(qb) : 0u;
-#pragma unroll
-                for (int o = kTop; o >= 1; o >>= 1) {
-                    ra |= (uint32_t)__shfl_xor((int)ra, o, 64);
-                    rb |= (uint32_t)__shfl_xor((int)rb, o, 64);
-                }
-                root_a = g.bcast0(ra);
-                root_b = g.bcast0(rb);
+                root_a = g.or_low32(r < (uint32_t)n ? 1u << label(qa) : 0u);
+                root_b = g.or_low32(r < (uint32_t)n ? 1u << label(qb) : 0u);
             }
             words = 1u << (P.narrow_bits - 5);
         } else {
