@@ -265,7 +265,12 @@ int ssw_context::launch_class(const DeviceBatch &db, const ClassSpec &cls, bool 
         mid.in_queue = cls.spill_queue[0];
         mid.out_list = cls.spill_list[1];
         mid.out_queue = cls.spill_queue[1];
-        rc = launch_pass(db, cls, t, false, mid);
+        // few automata per claim: the listed automata are fewer and their
+        // searches longer and more uneven (C_n among them) than the first
+        // launch's, so 16-automaton claims leave most groups idle
+        ssw_tuning tm = t;
+        tm.chunk = std::min(t.chunk, 4);
+        rc = launch_pass(db, cls, tm, false, mid);
         if (rc) return rc;
         last.in_list = cls.spill_list[1];
         last.in_queue = cls.spill_queue[1];
