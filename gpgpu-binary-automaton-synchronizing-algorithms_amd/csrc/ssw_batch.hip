@@ -34,6 +34,7 @@ namespace ssw {
 typedef __attribute__((address_space(3))) uint32_t lds_u32;
 typedef __attribute__((address_space(3))) uint64_t lds_u64;
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 typedef __attribute__((address_space(3))) u32x4 lds_u4;
 typedef __attribute__((address_space(3))) uint16_t lds_u16;
 typedef __attribute__((address_space(3))) uint8_t lds_u8;
@@ -1161,7 +1162,7 @@ struct SmallSolver {
         for (int t = 0; t < TT; ++t) {
             const uint32_t p = p0 + t * 64u + lane;
             v[t] = p < hi;
-            s[t] = rget<LDS>(v[t] ? p : p0);
+            s[t] = rget<LDS>(min(p, hi - 1u));
         }
         uint32_t img[TT];
 #pragma unroll
@@ -1209,9 +1210,10 @@ struct SmallSolver {
     // their test-and-sets, nw = the image is new, and the wavefront ballots
     // m[2t], m[2t+1] (new images under a, b) and sm (a singleton image).
     // Every ballot is of a plain compare, so the lane masks are used as the
-    // compares produce them.  A lane past the level repeats entry rd (or the
-    // chunk's first entry): its singleton test is that of the lane owning that
-    // entry, and it skips the test-and-sets (its results read as not new).
+    // compares produce them.  A lane past the level repeats the level's last
+    // entry hi - 1, which then lies in the same step (or chunk): its singleton
+    // test is that of the lane owning that entry, and it skips the
+    // test-and-sets (its results read as not new).
     // The halves of y = x & (x - 0x10001) are a & (a - 1) and b & (b - 1) of
     // the nonempty images a (low half) and b (high half).  Q is never a
     // singleton here (n = 1 ends at start), so a singleton image is always
@@ -1225,7 +1227,10 @@ struct SmallSolver {
         for (int t = 0; t < TT; ++t) {
             const uint32_t x = img[t];
             const uint32_t y = x & (x - 0x00010001u);
-            sm |= __ballot((y & 0xFFFFu) == 0u) | __ballot(y < 0x10000u);
+            // a half of y is zero: min of the halves (one v_min_u16 with a
+            // word select) is zero
+            const u16x2 yh = __builtin_bit_cast(u16x2, y);
+            sm |= __ballot(__builtin_elementwise_min(yh, yh.yx).x == 0);
             const uint32_t ba = 1u << (x & 31u), bb = 1u << ((x >> 16) & 31u);
             uint32_t oa = ~0u, ob = ~0u;
             if (v[t]) {
@@ -1250,9 +1255,10 @@ struct SmallSolver {
         for (int t = 0; t < TT; ++t) {
             const uint32_t p = rd + t * G + r;
             v[t] = p < hi;
-            // a lane past the level repeats entry rd: its singleton test is
-            // that of the lane owning rd, and it skips the test-and-sets
-            s[t] = rget<LDS>(v[t] ? p : rd);
+            // a lane past the level repeats entry hi - 1 (in this step): its
+            // singleton test is that of the lane owning it, and it skips the
+            // test-and-sets
+            s[t] = rget<LDS>(min(p, hi - 1u));
         }
         uint32_t img[TT];
 #pragma unroll

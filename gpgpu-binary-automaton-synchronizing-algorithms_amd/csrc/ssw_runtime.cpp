@@ -347,7 +347,8 @@ int ssw_context::launch_pass(const DeviceBatch &db, const ClassSpec &cls, const 
     const size_t per_group = batch_group_lds(variant, bm_words, arena_lds);
     P.lds_group_bytes = (uint32_t)per_group;
     // workgroup size: the tuned one, or (block_threads = 0) the one that keeps
-    // the most groups resident per CU under the LDS budget
+    // the most groups resident per CU under the LDS budget, the smallest of
+    // those (a small batch then spreads over more CUs)
     int per_cu = 0;
     if (!plain && G <= 64 && t.block_threads == 0) {
         int best_groups = 0;
@@ -357,7 +358,7 @@ int ssw_context::launch_pass(const DeviceBatch &db, const ClassSpec &cls, const 
             if (l > lds_per_block) continue;
             int occ = 0;
             HIP_TRY(batch_occupancy(variant, G, cand, l, &occ), "occupancy");
-            if (occ * (cand / G) >= best_groups && occ > 0) {
+            if (occ * (cand / G) > best_groups && occ > 0) {
                 best_groups = occ * (cand / G);
                 block = cand;
                 per_cu = occ;
