@@ -786,6 +786,12 @@ struct SmallSolver {
     const BatchParams &P;
     Grp g;
     uint32_t tab_a, stage_a, pimg_a, ctl_a, ring_a, bm_a;  // LDS byte addresses
+    // Lanes past the level take their test-and-sets to one of kDummies
+    // all-ones words (the staging area once an automaton has started; an OR
+    // leaves them all ones), so their results read as not new without a
+    // branch around the atomics.
+    static constexpr uint32_t kDummies = G >= 32 ? 32u : 16u;
+    uint32_t dum_a;
     glb_u16 *gring;
     uint32_t rmask, rcap;
     bool in_lds;
@@ -822,6 +828,7 @@ struct SmallSolver {
         gring = (glb_u16 *)(slot + P.garena_off);
         rcap = P.lds_arena_cap;
         rmask = rcap - 1u;
+        dum_a = stage_a + 4u * (g.rank() & (kDummies - 1u));
         narrow = P.narrow_bits > 0;
         it = 0;
         it_end = 0;
@@ -1005,6 +1012,9 @@ struct SmallSolver {
                 for (uint32_t k = 0; k < 6u; ++k) *l32(ctl_a + 4u * k) = 0u;
             }
         }
+        // the staging words are read by now: they become the all-ones dummy
+        // words of the test-and-sets
+        if (r < kDummies) *l32(stage_a + 4u * r) = ~0u;
         lsync();
         rd = 0;
         level = 0;
@@ -1034,13 +1044,21 @@ struct SmallSolver {
         if constexpr (G > 64) return step_level();
         if constexpr (T == 1 && G == 64) {
             // one entry per lane, one wavefront, LDS ring: the steps of the
-            // search run in one tight loop with a single path (the launch that
-            // takes every automaton spends most of its instructions here);
-            // the general path below takes over when the ring is short
+            // search run in one tight loop with a single path and a single
+            // exit test, its bookkeeping in selects (the launch that takes
+            // every automaton spends most of its instructions here); the
+            // general path below takes over when the ring is short
             if (in_lds) {
-                while (wr + 2u * G - rd <= rcap) {
-                    const int c = step_in<true, 1>();
-                    if (c != kContinue) return c;
+                const uint32_t lim = rcap - 2u * G;
+                while (wr - rd <= lim) {
+                    if (step_body<true, 1>()) return kFound;
+                    rd += G;
+                    if (rd >= hi) {
+                        ++level;
+                        rd = hi;
+                        hi = wr;
+                        if (rd == hi) return kNoSync;
+                    }
                 }
             }
         } else if constexpr (G == 64) {
@@ -1212,8 +1230,9 @@ struct SmallSolver {
     // Every ballot is of a plain compare, so the lane masks are used as the
     // compares produce them.  A lane past the level repeats the level's last
     // entry hi - 1, which then lies in the same step (or chunk): its singleton
-    // test is that of the lane owning that entry, and it skips the
-    // test-and-sets (its results read as not new).
+    // test is that of the lane owning that entry, and its test-and-sets go to
+    // an all-ones dummy word (results not new).  With no branch around them,
+    // the test-and-sets of all TT entries are issued before the first wait.
     // The halves of y = x & (x - 0x10001) are a & (a - 1) and b & (b - 1) of
     // the nonempty images a (low half) and b (high half).  Q is never a
     // singleton here (n = 1 ends at start), so a singleton image is always
@@ -1222,6 +1241,19 @@ struct SmallSolver {
     template <int TT>
     __device__ __forceinline__ void expand(const uint32_t (&img)[TT], const bool (&v)[TT], bool (&nw)[2 * TT],
                                            uint64_t (&m)[2 * TT], uint64_t &sm) const {
+        uint32_t oa[TT], ob[TT], ba[TT], bb[TT];
+#pragma unroll
+        for (int t = 0; t < TT; ++t) {
+            const uint32_t x = img[t];
+            ba[t] = 1u << (x & 31u);
+            bb[t] = 1u << ((x >> 16) & 31u);
+            // a before b, so a == b is counted once; word of a: bits 5..15 of
+            // x, of b: bits 21..31
+            oa[t] = or_wg(l32(v[t] ? (bm_a | ((x >> 3) & 0x1FFCu)) : dum_a), ba[t]);
+            ob[t] = or_wg(l32(v[t] ? (bm_a | ((x >> 19) & 0x1FFCu)) : dum_a), bb[t]);
+        }
+        // every test-and-set in flight before the first result is used
+        if constexpr (TT > 1) __builtin_amdgcn_sched_barrier(0);
         sm = 0;
 #pragma unroll
         for (int t = 0; t < TT; ++t) {
@@ -1231,23 +1263,17 @@ struct SmallSolver {
             // word select) is zero
             const u16x2 yh = __builtin_bit_cast(u16x2, y);
             sm |= __ballot(__builtin_elementwise_min(yh, yh.yx).x == 0);
-            const uint32_t ba = 1u << (x & 31u), bb = 1u << ((x >> 16) & 31u);
-            uint32_t oa = ~0u, ob = ~0u;
-            if (v[t]) {
-                // issued back to back; the second sees the first, so a == b
-                // is counted once; word of a: bits 5..15 of x, of b: bits 21..31
-                oa = or_wg(l32(bm_a | ((x >> 3) & 0x1FFCu)), ba);
-                ob = or_wg(l32(bm_a | ((x >> 19) & 0x1FFCu)), bb);
-            }
-            nw[2 * t] = (oa & ba) == 0u;
-            nw[2 * t + 1] = (ob & bb) == 0u;
+            nw[2 * t] = (oa[t] & ba[t]) == 0u;
+            nw[2 * t + 1] = (ob[t] & bb[t]) == 0u;
             m[2 * t] = __ballot(nw[2 * t]);
             m[2 * t + 1] = __ballot(nw[2 * t + 1]);
         }
     }
 
+    // One step: TT entries per lane from rd; appends the new images, advances
+    // wr; true when an image is a singleton.
     template <bool LDS, int TT>
-    __device__ int step_in() {
+    __device__ bool step_body() {
         const uint32_t r = g.rank();
         uint32_t s[TT];
         bool v[TT];
@@ -1256,8 +1282,7 @@ struct SmallSolver {
             const uint32_t p = rd + t * G + r;
             v[t] = p < hi;
             // a lane past the level repeats entry hi - 1 (in this step): its
-            // singleton test is that of the lane owning it, and it skips the
-            // test-and-sets
+            // singleton test is that of the lane owning it
             s[t] = rget<LDS>(min(p, hi - 1u));
         }
         uint32_t img[TT];
@@ -1280,7 +1305,12 @@ struct SmallSolver {
             }
         }
         wr = uni<G>(wr + total);
-        if (any_single) return kFound;
+        return any_single;
+    }
+
+    template <bool LDS, int TT>
+    __device__ int step_in() {
+        if (step_body<LDS, TT>()) return kFound;
         rd = uni<G>(rd + G * TT);
         if (rd >= hi) {
             if (G > 64 || !LDS)
