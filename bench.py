@@ -8,10 +8,10 @@ flags and expanded-subset counts.  `value` is automata/s of that step.
 Also reported on the same JSON line:
   * subsets expanded/s of the step, and the plain one-automaton-per-launch GPU
     BFS on a sample of the same batch (batched_vs_plain);
-  * the HBM roofline of the batched solve (two launches of the batched
-    kernel): algorithmic bytes 24*E + 2n + 4 per automaton over the kernel
-    time measured with HIP events on the solver's stream around both
-    launches; `traffic` from the committed rocprofv3 PMC summary;
+  * the HBM roofline of the batched solve (the batched kernel's launches of
+    one solve): algorithmic bytes 24*E + 2n + 4 per automaton over the
+    kernel time measured with HIP events on the solver's stream around all
+    of them; `traffic` from the committed rocprofv3 PMC summary;
   * cpu_baseline: the C restatement of the reference's SlimCPUUntweaked
     search (oracle/ssw_oracle.c) on a bounded sample, on the host cores;
   * the other workloads, each run once: plain_n10 (10k automata, n = 10, with
@@ -59,10 +59,10 @@ def algorithmic_bytes(n, expanded):
 
 
 def traffic_from_profile():
-    """HBM bytes of one solve of the batch (its two launches: every automaton
-    with a small LDS ring, then the ones whose frontier outgrew it), from the
-    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by a copy
-    kernel)."""
+    """HBM bytes of one solve of the batch (its launches: every automaton
+    with a small LDS ring, then the ones listed for a full bitmap or a larger
+    ring), from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+    (scaled by a copy kernel)."""
     if not os.path.exists(TRAFFIC):
         return None
     for line in open(TRAFFIC):
