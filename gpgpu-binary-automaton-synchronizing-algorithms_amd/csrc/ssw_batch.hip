@@ -1068,7 +1068,9 @@ struct SmallSolver {
                 while (wr + 2u * G * T - rd <= rcap) {
                     const uint32_t left = hi - rd;
                     int c;
-                    if (T >= 4 && left > 2u * G)
+                    if (T >= 8 && left > 4u * G)
+                        c = step_in<true, (T >= 8 ? 8 : 1)>();
+                    else if (T >= 4 && left > 2u * G)
                         c = step_in<true, (T >= 4 ? 4 : 1)>();
                     else if (T >= 2 && left > G)
                         c = step_in<true, (T >= 2 ? 2 : 1)>();
@@ -1083,14 +1085,24 @@ struct SmallSolver {
         // T = 4 step over 150..256 entries costs less than T = 2 plus T = 1
         // steps, and it saves their LDS round trips)
         const uint32_t left = hi - rd;
-        const uint32_t tt = (T >= 4 && left > 2u * G) ? 4u : ((T >= 2 && left > G) ? 2u : 1u);
-        // the ring holds the unread window plus two images per entry taken
-        if (in_lds && wr + 2u * G * tt - rd > rcap) {
-            if (P.spill) return spill();
-            migrate();
+        uint32_t tt = (T >= 8 && left > 4u * G)   ? 8u
+                      : (T >= 4 && left > 2u * G) ? 4u
+                                                  : ((T >= 2 && left > G) ? 2u : 1u);
+        // the ring holds the unread window plus two images per entry taken:
+        // fewer entries per lane while a wider step does not fit, then spill
+        // (or move to the HBM ring)
+        if (in_lds) {
+            while (tt > 1u && wr + 2u * G * tt - rd > rcap) tt >>= 1;
+            if (wr + 2u * G * tt - rd > rcap) {
+                if (P.spill) return spill();
+                migrate();
+            }
         }
         // one code path per ring location, so the LDS path carries no wait on
         // outstanding HBM loads (the prefetched next automaton)
+        if constexpr (T >= 8) {
+            if (tt == 8u) return in_lds ? step_in<true, 8>() : step_in<false, 8>();
+        }
         if constexpr (T >= 4) {
             if (tt == 4u) return in_lds ? step_in<true, 4>() : step_in<false, 4>();
         }
@@ -1433,7 +1445,8 @@ static hipError_t occ_t(int block, size_t lds, int *blocks) {
 }
 
 // variant: 0 = general (witness, HBM bitmaps); 1..3 = small path with T = 1, 2, 4
-// and nibble tables; 4..6 = small path with T = 1, 2, 4 and byte tables
+// and nibble tables; 4..6 = small path with T = 1, 2, 4 and byte tables; 7 =
+// small path with T = 8 and byte tables, one wavefront per automaton
 #define SSW_SMALL_CASES(FN, V, TT, BT, ...)                                                                   \
     case V * 1000 + 16: return FN<SmallSolver<WaveGroup<16>, TT, BT>, WaveGroup<16>>(__VA_ARGS__);            \
     case V * 1000 + 32: return FN<SmallSolver<WaveGroup<32>, TT, BT>, WaveGroup<32>>(__VA_ARGS__);            \
@@ -1452,6 +1465,7 @@ static hipError_t occ_t(int block, size_t lds, int *blocks) {
         SSW_SMALL_CASES(FN, 4, 1, true, __VA_ARGS__)                                       \
         SSW_SMALL_CASES(FN, 5, 2, true, __VA_ARGS__)                                       \
         SSW_SMALL_CASES(FN, 6, 4, true, __VA_ARGS__)                                       \
+        case 7064: return FN<SmallSolver<WaveGroup<64>, 8, true>, WaveGroup<64>>(__VA_ARGS__); \
         default: return hipErrorInvalidValue;                                              \
     }
 

@@ -293,11 +293,13 @@ int ssw_context::launch_pass(const DeviceBatch &db, const ClassSpec &cls, const 
     const bool want_wit = db.witness != nullptr;
     // the small path: bitmap in LDS, no witness, 16-bit ring frontier
     const bool small = io.force_small || (use_lds && !want_wit && nmax <= kMaxLdsN);
-    const int T = t.lane_entries >= 4 ? 4 : (t.lane_entries >= 2 ? 2 : 1);
     // the last launch of a spill chain takes the wide searches: byte-sliced
-    // image tables (two lookups per subset instead of four)
+    // image tables (two lookups per subset instead of four), and with one
+    // wavefront per automaton up to 8 entries per lane
     const bool byte_tab = io.byte_tab;
-    const int variant = small ? (T == 1 ? 1 : (T == 2 ? 2 : 3)) + (byte_tab ? 3 : 0) : 0;
+    const int T = (t.lane_entries >= 8 && byte_tab && G == 64) ? 8
+                  : (t.lane_entries >= 4 ? 4 : (t.lane_entries >= 2 ? 2 : 1));
+    const int variant = !small ? 0 : (T == 8 ? 7 : (T == 1 ? 1 : (T == 2 ? 2 : 3)) + (byte_tab ? 3 : 0));
 
     BatchParams P;
     std::memset(&P, 0, sizeof P);
@@ -991,7 +993,7 @@ int ssw_set_tuning(ssw_context *ctx, int32_t n, const ssw_tuning *t) {
     if (t->spill_arena_lds < 0 || t->spill_arena_lds > 65536)
         return ctx->fail(SSW_ERR_ARG, "spill_arena_lds %d", t->spill_arena_lds);
     if (t->spill_lane_entries != 0 && t->spill_lane_entries != 1 && t->spill_lane_entries != 2 &&
-        t->spill_lane_entries != 4)
+        t->spill_lane_entries != 4 && t->spill_lane_entries != 8)
         return ctx->fail(SSW_ERR_ARG, "spill_lane_entries %d", t->spill_lane_entries);
     const int SG = t->spill_group_lanes;
     if (SG != 0 && SG != 16 && SG != 32 && SG != 64 && SG != 128 && SG != 256)

@@ -109,12 +109,12 @@ int tune_one(ssw_context &ctx, int n, const DeviceBatch &db, Timed *out) {
                 return rc;
             if ((rc = sweep(ctx, n, db, best, {1, 2, 4}, [](ssw_tuning &t, int v) { t.lane_entries = v; })))
                 return rc;
-            if ((rc = sweep(ctx, n, db, best, {1, 2, 4}, [](ssw_tuning &t, int v) { t.spill_lane_entries = v; })))
+            if ((rc = sweep(ctx, n, db, best, {1, 2, 4, 8}, [](ssw_tuning &t, int v) { t.spill_lane_entries = v; })))
                 return rc;
             if ((rc = sweep(ctx, n, db, best, {0, 128, 256}, [](ssw_tuning &t, int v) { t.spill_group_lanes = v; })))
                 return rc;
             if (best.t.spill_group_lanes > 64 &&
-                (rc = sweep(ctx, n, db, best, {1, 2, 4}, [](ssw_tuning &t, int v) { t.spill_lane_entries = v; })))
+                (rc = sweep(ctx, n, db, best, {1, 2, 4, 8}, [](ssw_tuning &t, int v) { t.spill_lane_entries = v; })))
                 return rc;
             if (n > 8 && (rc = sweep(ctx, n, db, best, {0, n - 1, n - 2}, [](ssw_tuning &t, int v) {
                               t.narrow_bits = v >= 8 ? v : 0;

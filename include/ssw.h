@@ -149,7 +149,8 @@ typedef struct ssw_tuning {
     int32_t spill_arena_lds;  /* n <= 16: 0 = a frontier that outgrows the LDS ring moves to HBM;
                                  else such automata are re-solved by a second launch whose LDS
                                  ring holds this many entries */
-    int32_t spill_lane_entries; /* frontier entries per lane in that second launch: 1, 2 or 4 */
+    int32_t spill_lane_entries; /* frontier entries per lane in that second launch: 1, 2, 4, or 8
+                                   (8 with one wavefront per automaton, else taken as 4) */
     int32_t spill_group_lanes;  /* threads per automaton in that second launch: 0 (= group_lanes),
                                    16, 32, 64, 128 or 256 */
     int32_t narrow_bits;        /* n <= 16 with spill on: 0, or 8..n-1 = the first launch relabels the

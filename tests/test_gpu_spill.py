@@ -60,6 +60,8 @@ CONFIGS = [
     (256, 256, 2, 1024, 4, 0, 0, 0),        # no spill: level-split groups with HBM relocation
     (64, 0, 2, 2048, 4, 0, 0, 0),           # no spill: one wavefront, frontier moves to HBM
     (64, 128, 16, 512, 1, 4096, 4, 0, 15),  # relabelled states, 2^15-bit bitmap, three launches
+    (64, 128, 16, 512, 1, 4096, 8, 0, 15),  # eight entries per lane in the last launch
+    (64, 0, 4, 256, 1, 2048, 8, 0),         # eight entries per lane, ring too small for C_16's widest steps
     (64, 128, 16, 512, 1, 4096, 4, 0, 13),
     (32, 0, 8, 512, 1, 4096, 4, 0, 14),
     (256, 256, 2, 1024, 2, 4096, 4, 0, 14), # relabelled, four wavefronts per automaton
