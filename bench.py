@@ -155,13 +155,18 @@ def bench_verify_n13(ctx, count=1 << 21):
     System.Random twin (seed 1234567 + count, Program.cs:59,94), timed end to
     end from host arrays as Program.cs:137-143 times Verify."""
     A, B = ssw.generate_dotnet(1234567 + count, 13, count)
-    ctx.verify(13, A[:4096], B[:4096])
-    t0 = time.perf_counter()
-    bad = ctx.verify(13, A, B)
-    dt = time.perf_counter() - t0
+    ctx.verify(13, A, B)  # the context's staging and device buffers reach this size once
+    ts = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        bad = ctx.verify(13, A, B)
+        ts.append(time.perf_counter() - t0)
+    dt = float(np.median(ts))
     return {"workload": f"Verify of {count} random automata n=13 (System.Random twin, seed 1234567+count), "
-                        "host arrays in, violating index out (PCIe included)",
-            "automata_verified_per_s": count / dt, "ms": dt * 1e3, "violation": bad,
+                        "host arrays in, violating index out (PCIe included); median of 3 calls after one "
+                        "of the same size",
+            "automata_verified_per_s": count / dt, "ms": dt * 1e3, "kernel_ms": ctx.last_kernel_time()[0],
+            "violation": bad,
             "reference_published_n13": "SlimGPUQueue 1,953,926 automata/s at 2,097,152 problems "
                                        "(GTX 1060, PDF p.18); not the headline metric"}
 

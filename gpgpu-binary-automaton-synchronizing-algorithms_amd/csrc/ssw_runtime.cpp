@@ -785,63 +785,119 @@ int ssw_context::solve_entry(const ssw_batch *b, int32_t memory, int mode, int64
     return rc;
 }
 
+// Packs automata [lo, hi) of the int32 arrays into the staging bytes,
+// checking every transition; returns the first bad automaton or -1.
+static int64_t pack_range(int32_t n, const int32_t *A, const int32_t *B, uint8_t *packed, int64_t lo, int64_t hi) {
+    for (int64_t i = lo; i < hi; ++i) {
+        uint8_t *d = packed + (size_t)i * 2 * n;
+        bool ok = true;
+        for (int q = 0; q < n; ++q) {
+            const int32_t x = A[i * n + q], y = B[i * n + q];
+            ok &= (uint32_t)x < (uint32_t)n && (uint32_t)y < (uint32_t)n;
+            d[q] = (uint8_t)x;
+            d[n + q] = (uint8_t)y;
+        }
+        if (!ok) return i;
+    }
+    return -1;
+}
+
 int ssw_context::compute_i32(int32_t n, int64_t count, const int32_t *A, const int32_t *B,
                              int32_t *length_out, uint8_t *sync_out, uint8_t *witness_out,
                              int32_t witness_stride, int64_t *expanded_out, int32_t bound,
                              int64_t *violation_out) {
     if (n < 1 || n > SSW_MAX_N) return fail(SSW_ERR_ARG, "n=%d out of [1,30]", n);
     if (count < 0 || (count > 0 && (!A || !B))) return fail(SSW_ERR_ARG, "null argument");
+    if (violation_out) *violation_out = -1;
     // flatten like SlimGPUQueue.cs:108-126, packed to bytes in pinned staging
-    // memory by up to 16 host threads, checked on the way
+    // memory by host threads, checked on the way.  Large calls are cut into
+    // chunks: the threads pack chunk k + 1 while chunk k is copied and solved,
+    // so the host work overlaps the transfer and the kernels.
     int rc = ensure_stage((size_t)count * 2 * n);
     if (rc) return rc;
     uint8_t *packed = (uint8_t *)h_stage;
-    std::mutex m;
-    int64_t bad = -1;
-    parallel_ranges(count, 1 << 15, [&](int64_t lo, int64_t hi) {
-        int64_t first = -1;
-        for (int64_t i = lo; i < hi; ++i) {
-            uint8_t *d = packed + (size_t)i * 2 * n;
-            bool ok = true;
-            for (int q = 0; q < n; ++q) {
-                const int32_t x = A[i * n + q], y = B[i * n + q];
-                ok &= (uint32_t)x < (uint32_t)n && (uint32_t)y < (uint32_t)n;
-                d[q] = (uint8_t)x;
-                d[n + q] = (uint8_t)y;
-            }
-            if (!ok) {
-                first = i;
-                break;
-            }
-        }
-        if (first >= 0) {
-            std::lock_guard<std::mutex> lk(m);
-            if (bad < 0 || first < bad) bad = first;
-        }
-    });
-    if (bad >= 0) {
-        int q = 0;
-        while (q < n && (uint32_t)A[bad * n + q] < (uint32_t)n && (uint32_t)B[bad * n + q] < (uint32_t)n) ++q;
-        return fail(SSW_ERR_ARG, "automaton %lld state %d: transition out of [0,%d)", (long long)bad, q, n);
+    constexpr int64_t kMinChunk = 1 << 17;
+    const int64_t nchunks = std::max<int64_t>(1, std::min<int64_t>(8, count / kMinChunk));
+    auto chunk_lo = [&](int64_t k) { return count * k / nchunks; };
+    const int64_t hw = std::max<int64_t>(1, (int64_t)std::thread::hardware_concurrency());
+    const int workers = (int)std::max<int64_t>(1, std::min<int64_t>(hw, 16) - (nchunks > 1 ? 1 : 0));
+    std::vector<std::atomic<int>> packed_parts(nchunks);
+    std::vector<std::atomic<int64_t>> bad(nchunks);
+    for (int64_t k = 0; k < nchunks; ++k) {
+        packed_parts[k].store(0);
+        bad[k].store(-1);
     }
+    // device buffers at the largest chunk's size before the pipeline starts
+    {
+        const uint64_t most = (uint64_t)((count + nchunks - 1) / nchunks);
+        if ((rc = ensure(d_delta, most * 2u * (uint64_t)n, "delta"))) return rc;
+        if ((rc = ensure(d_len, 4ull * most, "length"))) return rc;
+        if (sync_out && (rc = ensure(d_sync, most, "sync"))) return rc;
+        if (expanded_out && (rc = ensure(d_exp, 8ull * most, "expanded"))) return rc;
+        if (witness_out && (rc = ensure(d_wit, (uint64_t)witness_stride * most, "witness"))) return rc;
+    }
+    std::atomic<bool> abort_pack{false};
+    auto work = [&](int t) {
+        for (int64_t k = 0; k < nchunks && !abort_pack.load(std::memory_order_relaxed); ++k) {
+            const int64_t lo = chunk_lo(k), len = chunk_lo(k + 1) - lo;
+            const int64_t first = pack_range(n, A, B, packed, lo + len * t / workers, lo + len * (t + 1) / workers);
+            if (first >= 0) {
+                int64_t cur = bad[k].load();
+                while ((cur < 0 || first < cur) && !bad[k].compare_exchange_weak(cur, first)) {
+                }
+            }
+            packed_parts[k].fetch_add(1, std::memory_order_release);
+        }
+    };
+    std::vector<std::thread> ts;
+    if (workers > 1 || nchunks > 1)
+        for (int t = 0; t < workers; ++t) ts.emplace_back(work, t);
+    else
+        work(0);
+    auto join_all = [&] {
+        for (auto &th : ts) th.join();
+        ts.clear();
+    };
     static int32_t no_length;
-    ssw_batch b;
-    std::memset(&b, 0, sizeof b);
-    b.count = count;
-    b.uniform_n = n;
-    b.delta = packed;
-    b.length = length_out ? length_out : &no_length;
-    b.sync = sync_out;
-    b.expanded = expanded_out;
-    b.witness = witness_out;
-    b.witness_stride = witness_stride;
-    b.bound = bound;
-    host_checked = true;
-    host_no_length = length_out == nullptr;
-    rc = solve_entry(&b, SSW_HOST, kModeBatch, violation_out);
-    host_checked = false;
-    host_no_length = false;
-    return rc;
+    int64_t viol = -1;
+    for (int64_t k = 0; k < nchunks; ++k) {
+        while (packed_parts[k].load(std::memory_order_acquire) < workers) std::this_thread::yield();
+        const int64_t first = bad[k].load();
+        if (first >= 0) {
+            abort_pack.store(true);
+            join_all();
+            int q = 0;
+            while (q < n && (uint32_t)A[first * n + q] < (uint32_t)n && (uint32_t)B[first * n + q] < (uint32_t)n) ++q;
+            return fail(SSW_ERR_ARG, "automaton %lld state %d: transition out of [0,%d)", (long long)first, q, n);
+        }
+        const int64_t lo = chunk_lo(k), len = chunk_lo(k + 1) - lo;
+        ssw_batch b;
+        std::memset(&b, 0, sizeof b);
+        b.count = len;
+        b.uniform_n = n;
+        b.delta = packed + (size_t)lo * 2 * n;
+        b.length = length_out ? length_out + lo : &no_length;
+        b.sync = sync_out ? sync_out + lo : nullptr;
+        b.expanded = expanded_out ? expanded_out + lo : nullptr;
+        b.witness = witness_out ? witness_out + (size_t)lo * witness_stride : nullptr;
+        b.witness_stride = witness_stride;
+        b.bound = bound;
+        host_checked = true;
+        host_no_length = length_out == nullptr;
+        int64_t v = -1;
+        rc = solve_entry(&b, SSW_HOST, kModeBatch, violation_out ? &v : nullptr);
+        host_checked = false;
+        host_no_length = false;
+        if (rc) {
+            abort_pack.store(true);
+            join_all();
+            return rc;
+        }
+        if (v >= 0 && (viol < 0 || lo + v < viol)) viol = lo + v;
+    }
+    join_all();
+    if (violation_out) *violation_out = viol;
+    return SSW_OK;
 }
 
 // ---------------------------------------------------------------------------

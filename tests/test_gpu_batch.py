@@ -130,3 +130,44 @@ def test_repeatable(ctx):
     r2 = ctx.solve_packed(delta.reshape(-1), len(delta), n, witness=True)
     np.testing.assert_array_equal(r1.length, r2.length)
     np.testing.assert_array_equal(r1.witness, r2.witness)
+
+
+def test_host_arrays_chunked(ctx):
+    """Host int32 arrays large enough to be packed and solved in chunks (the
+    packing of chunk k + 1 overlaps chunk k's transfer and kernels): every
+    output equals the oracle, witnesses included, and the violating index is
+    the smallest over all chunks."""
+    n = 12
+    count = 300_000
+    A, B = ssw.generate_uniform(4242, n, count)
+    L, S, E, v = O.slim(n, A, B, bound=9, threads=8)
+    r = ctx.compute(n, A, B, witness=True, bound=9)
+    np.testing.assert_array_equal(r.length, L)
+    np.testing.assert_array_equal(r.sync, S)
+    np.testing.assert_array_equal(r.expanded, E)
+    assert r.violation == v == int(np.nonzero(L > 9)[0][0])
+    for i in range(0, count, 9973):
+        if L[i] > 0:
+            w = ssw.witness_bits_to_word(r.witness[i], int(L[i]))
+            assert O.verify_word(n, A[i], B[i], w)
+    # the first violation only in the last chunk
+    late = np.nonzero(L > 9)[0]
+    late = int(late[late > count - 20_000][0])
+    A2, B2 = A.copy(), B.copy()
+    for i in np.nonzero(L > 9)[0]:
+        if i != late:
+            A2[i] = 0  # a constant letter: length 1
+    assert ctx.verify(n, A2, B2, bound=9) == late
+    assert ctx.verify(n, A2, B2) == -1
+
+
+def test_host_arrays_bad_transition_in_last_chunk(ctx):
+    n = 10
+    A, B = ssw.generate_uniform(77, n, 300_000)
+    B[299_990, 4] = n
+    with pytest.raises(ssw.SswError, match="automaton 299990 state 4"):
+        ctx.compute(n, A, B)
+    # the context stays usable
+    r = ctx.compute(n, A[:1000], B[:1000])
+    L, _, _, _ = O.slim(n, A[:1000], B[:1000])
+    np.testing.assert_array_equal(r.length, L)
