@@ -142,8 +142,17 @@ def bench_large(ctx):
     r = ctx.solve_mixed(ns, [it[1] for it in items], [it[2] for it in items], mode="large")
     dt = time.perf_counter() - t0
     cerny_ok = all(int(r.length[i]) == (n - 1) ** 2 for i, (n, a, b, k) in enumerate(items) if k == "cerny")
+    # the HBM bitmaps' test-and-sets (two per expanded subset, device scope)
+    # over the wall time, against the device's ceiling for random ones
+    hbm_ceiling = ctx.measure_atomics("hbm")
+    tas = 2.0 * float(r.expanded.sum()) / dt
     return {"workload": "Cerny C_24..C_30 + 64 random n=30 (seed 1004), all concurrent, device-wide path",
             "wall_s": dt, "subsets": int(r.expanded.sum()), "subsets_per_s": float(r.expanded.sum()) / dt,
+            "atomics": {"where": "HBM", "achieved_per_s": tas, "ceiling_per_s": hbm_ceiling,
+                        "frac": tas / hbm_ceiling,
+                        "note": "two test-and-sets per expanded subset over the wall time (host loop included); "
+                                "ceiling from ssw_measure_atomics (random device-scope atomicOr-with-return "
+                                "into a 128 MiB bitmap)"},
             "cerny_lengths_equal_(n-1)^2": cerny_ok,
             "cerny_expanded_equal_2^n-n-1": all(int(r.expanded[i]) == 2 ** n - n - 1
                                                for i, (n, a, b, k) in enumerate(items) if k == "cerny"),
@@ -165,7 +174,7 @@ def bench_verify_n13(ctx, count=1 << 21):
     return {"workload": f"Verify of {count} random automata n=13 (System.Random twin, seed 1234567+count), "
                         "host arrays in, violating index out (PCIe included); median of 3 calls after one "
                         "of the same size",
-            "automata_verified_per_s": count / dt, "ms": dt * 1e3, "kernel_ms": ctx.last_kernel_time()[0],
+            "automata_verified_per_s": count / dt, "ms": dt * 1e3,
             "violation": bad,
             "reference_published_n13": "SlimGPUQueue 1,953,926 automata/s at 2,097,152 problems "
                                        "(GTX 1060, PDF p.18); not the headline metric"}
@@ -276,6 +285,16 @@ def main():
     abytes = algorithmic_bytes(np.full(count, n), expanded)
     achieved = abytes / (kernel_ms * 1e-3) / 1e9
     traffic = traffic_from_profile()
+    # test-and-sets: two per expanded subset (images under a and b), in LDS;
+    # against the device's own ceiling for random test-and-sets in 8 KiB
+    # LDS bitmaps (every wavefront busy with nothing else)
+    lds_ceiling = ctx.measure_atomics("lds")
+    tas = 2.0 * subsets / (kernel_ms * 1e-3)
+    atomics = {"where": "LDS", "achieved_per_s": tas, "ceiling_per_s": lds_ceiling,
+               "frac": tas / lds_ceiling,
+               "note": "two test-and-sets per expanded subset over the kernel time; ceiling from "
+                       "ssw_measure_atomics (random atomicOr-with-return into an 8 KiB LDS bitmap per wavefront)"}
+    log(f"LDS test-and-sets {tas:.3e}/s of a {lds_ceiling:.3e}/s ceiling")
 
     cpu = None
     parity = None
@@ -319,6 +338,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
         "cpu_baseline": cpu,
+        "atomics": atomics,
     }
     if not args.no_extras:
         for name, fn in (("plain_n10", lambda: bench_plain_n10(ctx)), ("uneven_12_24", lambda: bench_uneven(ctx)),

@@ -216,4 +216,5 @@ struct ssw_context {
     int read_progress(ssw_progress *out);
     void start_progress_thread();
     void stop_progress_thread();
+    int measure_atomics(int32_t kind, double *ops_per_s);
 };

@@ -31,6 +31,7 @@ EXPORTS = [
     "ssw_copy_to_host", "ssw_copy_on_device", "ssw_synchronize", "ssw_last_kernel_time", "ssw_generate_uniform",
     "ssw_generate_cerny", "ssw_generate_dotnet", "ssw_server_create", "ssw_server_submit",
     "ssw_server_poll", "ssw_server_wait", "ssw_server_stats_get", "ssw_server_destroy",
+    "ssw_measure_atomics",
 ]
 
 
@@ -135,6 +136,7 @@ def lib():
             "ssw_server_wait": ([vp, u64, i32, ctypes.POINTER(Result)], ctypes.c_int),
             "ssw_server_stats_get": ([vp, ctypes.POINTER(ServerStats)], ctypes.c_int),
             "ssw_server_destroy": ([vp], ctypes.c_int),
+            "ssw_measure_atomics": ([vp, i32, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -346,6 +348,14 @@ class Context:
 
     def synchronize(self):
         self._check(self._L.ssw_synchronize(self.h), "synchronize")
+
+    def measure_atomics(self, kind):
+        """Test-and-set ceiling of the device in operations/s: kind "lds" (8 KiB
+        bitmap per wavefront) or "hbm" (128 MiB bitmap, device scope)."""
+        ops = ctypes.c_double(0.0)
+        self._check(self._L.ssw_measure_atomics(self.h, {"lds": 0, "hbm": 1}[kind], ctypes.byref(ops)),
+                    "measure_atomics")
+        return ops.value
 
     def last_kernel_time(self):
         ms = ctypes.c_double()

@@ -201,6 +201,16 @@ int ssw_synchronize(ssw_context *ctx);
    events on the context's stream, and its launch count. */
 int ssw_last_kernel_time(ssw_context *ctx, double *ms, int64_t *launches);
 
+/* ---- measurement --------------------------------------------------------- */
+
+/* Test-and-set ceiling of the device (no counterpart in the reference; for the
+   roofline report beside the solvers' own rates): every wavefront of the GPU
+   issues atomicOr-with-return at random bits, either of an 8 KiB LDS bitmap
+   per wavefront (the n <= 16 visited set) or of a 128 MiB HBM bitmap at device
+   scope (the n = 30 visited set).  Writes operations per second. */
+enum { SSW_ATOMICS_LDS = 0, SSW_ATOMICS_HBM = 1 };
+int ssw_measure_atomics(ssw_context *ctx, int32_t kind, double *ops_per_s);
+
 /* ---- generators (the SharedDefinitions Problem.* generators) ------------ */
 
 /* Uniform i.i.d. transitions from a counter-based splitmix64 stream: value
