@@ -940,9 +940,11 @@ struct SmallSolver {
             // Q).  The states are relabelled so that U takes the |U| lowest
             // labels; the visited bitmap then needs 2^|U| bits, and automata
             // with |U| above the bitmap's width go to the next launch.
-            // Automata with n > 16 (lanes 0..n-1, n <= 30, one wavefront) are
-            // solved this way too when |U| <= 16; their root Q does not fit
-            // the 16-bit ring, so its level is expanded here.
+            // The root level is expanded here: level 1 = {a(Q), b(Q)} comes
+            // from two more reductions, which costs less than the search's
+            // step over Q alone.  Automata with n > 16 (lanes 0..n-1,
+            // n <= 30, one wavefront) are solved this way too when |U| <= 16
+            // (their root Q would not fit the 16-bit ring anyway).
             const uint32_t U = g.or_low32(r < (uint32_t)n ? (1u << qa) | (1u << qb) : 0u);
             const uint32_t k = (uint32_t)__popc(U);
             if (k > (uint32_t)P.narrow_bits) return spill();
@@ -957,10 +959,8 @@ struct SmallSolver {
             } else if (r < 16u) {
                 *l32(pimg_a + 4u * r) = 0u;
             }
-            if (n > 16) {
-                root_a = g.or_low32(r < (uint32_t)n ? 1u << label(qa) : 0u);
-                root_b = g.or_low32(r < (uint32_t)n ? 1u << label(qb) : 0u);
-            }
+            root_a = g.or_low32(r < (uint32_t)n ? 1u << label(qa) : 0u);
+            root_b = g.or_low32(r < (uint32_t)n ? 1u << label(qb) : 0u);
             words = 1u << (P.narrow_bits - 5);
         } else {
             // singleton images of the 16 states, zero past n
@@ -994,8 +994,8 @@ struct SmallSolver {
             }
         }
         lsync();
-        // n > 16: level 1 = {a(Q), b(Q)} seeds the ring (Q counts once in E)
-        const bool pre = n > 16;
+        // relabelled: level 1 = {a(Q), b(Q)} seeds the ring (Q counts once in E)
+        const bool pre = narrow;
         if (r == 0) {
             // relabelled, Q lies outside the narrow bitmap and is never reached again
             if (!narrow) or_wg(l32(bm_a | ((Q >> 3) & 0x1FFCu)), 1u << (Q & 31u));
