@@ -903,10 +903,11 @@ struct SmallSolver {
     }
     __device__ bool has_next() const { return nx_ai >= 0; }
 
-    // Starts the prefetched automaton: -1 when its input is invalid (the
-    // result is written), kNoSync when n = 1 (Q is the only subset and
-    // already a singleton: not synchronizing, one subset expanded), else
-    // kContinue.
+    // Starts the prefetched automaton and expands its root level: -1 when its
+    // input is invalid (the result is written), kFound when a(Q) or b(Q) is
+    // a singleton, kNoSync when Q is the only reachable subset (both letters
+    // permutations, or n = 1: not synchronizing, one subset expanded), else
+    // kContinue with level 1 in the ring.
     __device__ int begin() { return start(); }
     __device__ int start() {
         ai = nx_ai;
@@ -965,6 +966,8 @@ struct SmallSolver {
         } else {
             // singleton images of the 16 states, zero past n
             if (r < 16u) *l32(pimg_a + 4u * r) = r < (uint32_t)n ? (1u << qa) | (1u << (16u + qb)) : 0u;
+            root_a = g.or_low32(r < (uint32_t)n ? 1u << qa : 0u);
+            root_b = g.or_low32(r < (uint32_t)n ? 1u << qb : 0u);
             words = n >= 5 ? (1u << (n - 5)) : 1u;
         }
         if (words >= 4) {
@@ -994,19 +997,19 @@ struct SmallSolver {
             }
         }
         lsync();
-        // relabelled: level 1 = {a(Q), b(Q)} seeds the ring (Q counts once in E)
-        const bool pre = narrow;
+        // The root level is expanded here: level 1 = the distinct subsets
+        // among a(Q), b(Q) other than Q (a permutation letter maps Q to
+        // itself) seeds the ring, and Q counts once in E.  With no new
+        // subset Q is the only reachable one: not synchronizing, E = 1 (this
+        // covers n = 1, where Q is already a singleton).
+        const bool na = root_a != Q, nb = root_b != Q && root_b != root_a;
         if (r == 0) {
             // relabelled, Q lies outside the narrow bitmap and is never reached again
             if (!narrow) or_wg(l32(bm_a | ((Q >> 3) & 0x1FFCu)), 1u << (Q & 31u));
-            if (pre) {
-                or_wg(l32(bm_a | ((root_a >> 3) & 0x1FFCu)), 1u << (root_a & 31u));
-                or_wg(l32(bm_a | ((root_b >> 3) & 0x1FFCu)), 1u << (root_b & 31u));
-                *l16(ring_a) = (uint16_t)root_a;
-                *l16(ring_a + 2u) = (uint16_t)root_b;
-            } else {
-                *l16(ring_a) = (uint16_t)Q;
-            }
+            if (na) or_wg(l32(bm_a | ((root_a >> 3) & 0x1FFCu)), 1u << (root_a & 31u));
+            if (nb) or_wg(l32(bm_a | ((root_b >> 3) & 0x1FFCu)), 1u << (root_b & 31u));
+            *l16(ring_a) = (uint16_t)(na ? root_a : root_b);
+            *l16(ring_a + 2u) = (uint16_t)root_b;
             if constexpr (G > 64) {
 #pragma unroll
                 for (uint32_t k = 0; k < 6u; ++k) *l32(ctl_a + 4u * k) = 0u;
@@ -1017,21 +1020,15 @@ struct SmallSolver {
         if (r < kDummies) *l32(stage_a + 4u * r) = ~0u;
         lsync();
         rd = 0;
-        level = 0;
-        e0 = 0;
-        if (pre) {
-            e0 = 1;
-            if (is_single(root_a) || is_single(root_b)) {
-                hi = wr = 0;
-                return kFound;  // length 1, one subset expanded
-            }
-            hi = wr = root_a == root_b ? 1u : 2u;
-            level = 1;
-            return kContinue;
+        e0 = 1;
+        if ((na && is_single(root_a)) || (nb && is_single(root_b))) {
+            hi = wr = 0;
+            level = 0;
+            return kFound;  // length 1, one subset expanded
         }
-        hi = 1;
-        wr = 1;
-        return n == 1 ? (int)kNoSync : (int)kContinue;
+        hi = wr = (na ? 1u : 0u) + (nb ? 1u : 0u);
+        level = 1;
+        return hi == 0u ? (int)kNoSync : (int)kContinue;
     }
 
     __device__ void migrate() {
@@ -1247,9 +1244,9 @@ struct SmallSolver {
     // the test-and-sets of all TT entries are issued before the first wait.
     // The halves of y = x & (x - 0x10001) are a & (a - 1) and b & (b - 1) of
     // the nonempty images a (low half) and b (high half).  Q is never a
-    // singleton here (n = 1 ends at start), so a singleton image is always
-    // new; it is still marked and appended, which is harmless because the
-    // search stops at this level.
+    // singleton here (n = 1 ends at start), and the search ends at the first
+    // singleton, so a singleton image is always new; it is still marked and
+    // appended, which is harmless because the search stops at this level.
     template <int TT>
     __device__ __forceinline__ void expand(const uint32_t (&img)[TT], const bool (&v)[TT], bool (&nw)[2 * TT],
                                            uint64_t (&m)[2 * TT], uint64_t &sm) const {
