@@ -217,4 +217,6 @@ struct ssw_context {
     void start_progress_thread();
     void stop_progress_thread();
     int measure_atomics(int32_t kind, double *ops_per_s);
+    std::map<std::tuple<int, int, int, size_t>, int> occ_cache;
+    hipError_t occupancy(int variant, int G, int block, size_t lds, int *out);
 };

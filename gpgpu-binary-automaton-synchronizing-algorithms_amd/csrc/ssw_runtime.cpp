@@ -359,7 +359,7 @@ int ssw_context::launch_pass(const DeviceBatch &db, const ClassSpec &cls, const 
             const size_t l = batch_block_lds(variant, G, cand / G, bm_words, arena_lds);
             if (l > lds_per_block) continue;
             int occ = 0;
-            HIP_TRY(batch_occupancy(variant, G, cand, l, &occ), "occupancy");
+            HIP_TRY(occupancy(variant, G, cand, l, &occ), "occupancy");
             if (occ * (cand / G) > best_groups && occ > 0) {
                 best_groups = occ * (cand / G);
                 block = cand;
@@ -412,7 +412,7 @@ int ssw_context::launch_pass(const DeviceBatch &db, const ClassSpec &cls, const 
 
     int grid = 1;
     if (!plain) {
-        if (per_cu == 0) HIP_TRY(batch_occupancy(variant, G, block, lds, &per_cu), "occupancy");
+        if (per_cu == 0) HIP_TRY(occupancy(variant, G, block, lds, &per_cu), "occupancy");
         if (per_cu < 1) return fail(SSW_ERR_ARG, "kernel does not fit a CU (lds %zu)", lds);
         grid = per_cu * cus;
         const int64_t need = (P.count + groups_per_block - 1) / groups_per_block;
@@ -590,10 +590,16 @@ int ssw_context::solve_device(const DeviceBatch &db, const uint8_t *host_n, bool
     }
     HIP_TRY(hipEventRecord(ev1, stream), "hipEventRecord");
 
-    // automata whose frontier outgrew a group's arena: device-wide path
-    uint32_t ndef = 0;
-    HIP_TRY(hipMemcpyAsync(&ndef, (uint8_t *)ctrs.p + kCtrDeferred, 4, hipMemcpyDeviceToHost, stream), "D2H");
+    // the deferred count and the violation in one read: automata whose
+    // frontier outgrew a group's arena go on to the device-wide path
+    static_assert(kCtrDeferred + 4 <= kCtrViolation && kCtrViolation + 8 <= 16, "counter layout");
+    alignas(8) uint8_t head[16];
+    HIP_TRY(hipMemcpyAsync(head, ctrs.p, sizeof head, hipMemcpyDeviceToHost, stream), "D2H");
     HIP_TRY(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    uint32_t ndef = 0;
+    unsigned long long viol = ~0ull;
+    std::memcpy(&ndef, head + kCtrDeferred, 4);
+    std::memcpy(&viol, head + kCtrViolation, 8);
     float ms = 0;
     HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1), "hipEventElapsedTime");
     last_ms = ms;
@@ -604,12 +610,25 @@ int ssw_context::solve_device(const DeviceBatch &db, const uint8_t *host_n, bool
         deferred_total += ndef;
         rc = large.solve_indices(*this, db, host_n, idx);
         if (rc) return rc;
+        HIP_TRY(hipMemcpyAsync(&viol, (uint8_t *)ctrs.p + kCtrViolation, 8, hipMemcpyDeviceToHost, stream), "D2H");
+        HIP_TRY(hipStreamSynchronize(stream), "hipStreamSynchronize");
     }
-    unsigned long long viol = ~0ull;
-    HIP_TRY(hipMemcpyAsync(&viol, (uint8_t *)ctrs.p + kCtrViolation, 8, hipMemcpyDeviceToHost, stream), "D2H");
-    HIP_TRY(hipStreamSynchronize(stream), "hipStreamSynchronize");
     if (violation_out) *violation_out = viol == ~0ull ? -1 : (int64_t)viol;
     return SSW_OK;
+}
+
+// Blocks of batch_kernel resident per CU for a launch shape, remembered per
+// shape (the runtime query costs more than a map lookup on every solve).
+hipError_t ssw_context::occupancy(int variant, int G, int block, size_t lds, int *out) {
+    const auto key = std::make_tuple(variant, G, block, lds);
+    auto it = occ_cache.find(key);
+    if (it != occ_cache.end()) {
+        *out = it->second;
+        return hipSuccess;
+    }
+    const hipError_t e = batch_occupancy(variant, G, block, lds, out);
+    if (e == hipSuccess) occ_cache[key] = *out;
+    return e;
 }
 
 // ---------------------------------------------------------------------------
