@@ -948,7 +948,14 @@ struct SmallSolver {
             // (their root Q would not fit the 16-bit ring anyway).
             const uint32_t U = g.or_low32(r < (uint32_t)n ? (1u << qa) | (1u << qb) : 0u);
             const uint32_t k = (uint32_t)__popc(U);
-            if (k > (uint32_t)P.narrow_bits) return spill();
+            if (k > (uint32_t)P.narrow_bits) {
+                if (P.skip_list && k == (uint32_t)n) {
+                    const uint32_t ia = g.or_low32(r < (uint32_t)n ? 1u << qa : 0u);
+                    const uint32_t ib = g.or_low32(r < (uint32_t)n ? 1u << qb : 0u);
+                    if (ia == Q || ib == Q) return spill_to(P.skip_queue, P.skip_list);
+                }
+                return spill_to(P.spill_queue, P.spill_list);
+            }
             const auto label = [U, k](uint32_t x) {
                 const uint32_t below = (uint32_t)__popc(U & ((1u << x) - 1u));
                 return ((U >> x) & 1u) ? below : k + x - below;
@@ -1225,10 +1232,11 @@ struct SmallSolver {
     }
 
     // Hands the automaton to the second launch (larger LDS ring).
-    __device__ int spill() {
+    __device__ int spill() { return spill_to(P.spill_queue, P.spill_list); }
+    __device__ int spill_to(uint32_t *queue, uint32_t *list) {
         if (g.rank() == 0) {
-            const uint32_t k = atomicAdd(P.spill_queue + seg * kQueueWords + 1, 1u);
-            P.spill_list[k] = (uint32_t)ai;
+            const uint32_t k = atomicAdd(queue + seg * kQueueWords + 1, 1u);
+            list[k] = (uint32_t)ai;
         }
         return kSpilled;
     }

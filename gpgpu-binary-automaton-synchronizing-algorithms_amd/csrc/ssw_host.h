@@ -70,6 +70,8 @@ struct PassIO {
     uint32_t *in_queue = nullptr;       // ... with this queue, filled by the launch before
     uint32_t *out_list = nullptr;       // automata handed to the next launch
     uint32_t *out_queue = nullptr;
+    uint32_t *skip_list = nullptr;      // ... or to the one after it (permutation letters)
+    uint32_t *skip_queue = nullptr;
     int narrow_bits = 0;                // relabelled states, 2^narrow_bits-bit bitmap
     bool byte_tab = false;
     bool force_small = false;           // the n <= 16 solver on a class of larger n (relabelled)

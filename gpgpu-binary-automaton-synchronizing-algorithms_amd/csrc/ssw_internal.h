@@ -97,6 +97,13 @@ struct BatchParams {
     uint32_t spill;
     uint32_t *spill_queue;
     uint32_t *spill_list;
+    // the launch after the next one (the large ring), or null: automata
+    // spilled at start whose letter a or b permutes the states (a(Q) or b(Q)
+    // is Q, as in every Cerny automaton) are handed there directly, since
+    // their searches are the long ones and the next launch would only replay
+    // their first levels
+    uint32_t *skip_queue;
+    uint32_t *skip_list;
     // n <= 16 path: > 0 relabels every automaton's states so that
     // U = a(Q) | b(Q) takes the lowest labels and uses a visited bitmap of
     // 2^narrow_bits bits; automata with |U| > narrow_bits are spilled at start

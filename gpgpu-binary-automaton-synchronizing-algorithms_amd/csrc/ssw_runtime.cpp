@@ -251,6 +251,11 @@ int ssw_context::launch_class(const DeviceBatch &db, const ClassSpec &cls, bool 
     first.out_list = cls.spill_list[0];
     first.out_queue = cls.spill_queue[0];
     first.narrow_bits = nb;
+    if (nb) {
+        // automata with a permutation letter go straight to the large ring
+        first.skip_list = cls.spill_list[1];
+        first.skip_queue = cls.spill_queue[1];
+    }
     int rc = launch_pass(db, cls, t, false, first);
     if (rc) return rc;
     PassIO last;
@@ -313,6 +318,8 @@ int ssw_context::launch_pass(const DeviceBatch &db, const ClassSpec &cls, const 
     P.spill = io.out_list ? 1u : 0u;
     P.spill_queue = io.out_queue;
     P.spill_list = io.out_list;
+    P.skip_queue = io.skip_queue;
+    P.skip_list = io.skip_list;
     P.narrow_bits = small ? io.narrow_bits : 0;
     P.length = db.length;
     P.sync = db.sync;
