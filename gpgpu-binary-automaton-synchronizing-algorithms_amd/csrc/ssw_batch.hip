@@ -772,7 +772,8 @@ struct GroupSolver {
 //
 // LDS (SmallLds): the bitmaps of all groups of the workgroup come first, each
 // aligned to its size, then one region per group (table, transitions
-// staging, singleton images, ring plus 64 dummy entries).  With aligned bases
+// staging, singleton images, control and spare words, ring plus 64 dummy
+// entries).  With aligned bases
 // a bitmap word or table entry address is one OR of the base and the scaled
 // index; lanes without a new entry write a dummy ring entry instead of
 // branching around the write.
@@ -787,10 +788,12 @@ struct SmallSolver {
     Grp g;
     uint32_t tab_a, stage_a, pimg_a, ctl_a, ring_a, bm_a;  // LDS byte addresses
     // Lanes past the level take their test-and-sets to one of kDummies
-    // all-ones words (the staging area once an automaton has started; an OR
-    // leaves them all ones), so their results read as not new without a
-    // branch around the atomics.
-    static constexpr uint32_t kDummies = G >= 32 ? 32u : 16u;
+    // all-ones words (once an automaton has started: the staging and
+    // singleton-image words, and for one wavefront per automaton also the
+    // control and spare words; an OR leaves them all ones), so their results
+    // read as not new without a branch around the atomics.  A wavefront's
+    // lanes get a word each: lanes sharing a word would serialise on it.
+    static constexpr uint32_t kDummies = G == 64 ? 64u : (G >= 32 ? 32u : 16u);
     uint32_t dum_a;
     glb_u16 *gring;
     uint32_t rmask, rcap;
@@ -824,7 +827,7 @@ struct SmallSolver {
         stage_a = tab_a + SmallLds::tab_bytes(BT);
         pimg_a = stage_a + 64;
         ctl_a = pimg_a + 64;
-        ring_a = ctl_a + 64;
+        ring_a = ctl_a + 128;
         gring = (glb_u16 *)(slot + P.garena_off);
         rcap = P.lds_arena_cap;
         rmask = rcap - 1u;

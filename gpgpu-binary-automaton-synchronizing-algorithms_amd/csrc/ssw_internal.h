@@ -36,7 +36,8 @@ struct SmallLds {
     }
     static __host__ __device__ inline uint32_t tab_bytes(bool byte_tab) { return byte_tab ? 2048u : 256u; }
     static __host__ __device__ inline uint32_t region_bytes(bool byte_tab, uint32_t ring_cap) {
-        const uint32_t b = tab_bytes(byte_tab) + 64u + 64u + 64u + 2u * (ring_cap + 64u);
+        // tables, staging, singleton images, control words, spare, ring (+64 dummy slots)
+        const uint32_t b = tab_bytes(byte_tab) + 64u + 64u + 64u + 64u + 2u * (ring_cap + 64u);
         const uint32_t a = byte_tab ? 1024u : 256u;
         return (b + a - 1u) / a * a;
     }

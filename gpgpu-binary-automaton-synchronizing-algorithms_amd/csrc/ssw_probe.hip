@@ -24,7 +24,12 @@ __device__ __forceinline__ uint32_t xorshift(uint32_t x) {
     return x;
 }
 
-// each wavefront owns 8 KiB of LDS and tests-and-sets random bits in it
+// Each wavefront owns 8 KiB of LDS and tests-and-sets random bits in it.  A
+// lane draws 8 random (word, bit) pairs once; iteration k XORs the word
+// indices with one wave-uniform value, which maps a random pattern to another
+// random one with the same bank conflicts, so an atomic costs two VALU
+// instructions (its address and the use of its result) and the rate is the
+// LDS's own.
 __global__ __launch_bounds__(kProbeThreads) void lds_probe_kernel(uint32_t iters, uint32_t *sink) {
     __shared__ uint32_t bm[kProbeThreads / 64][kLdsWords];
     uint32_t *flat = &bm[0][0];
@@ -32,11 +37,19 @@ __global__ __launch_bounds__(kProbeThreads) void lds_probe_kernel(uint32_t iters
     __syncthreads();
     uint32_t *mine = bm[threadIdx.x >> 6];
     uint32_t x = (blockIdx.x * kProbeThreads + threadIdx.x) * 2654435761u + 0x9E3779B9u;
-    uint32_t acc = 0;
-    for (uint32_t k = 0; k < iters; ++k) {
+    uint32_t word[8], bit[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
         x = xorshift(x);
-        acc += __hip_atomic_fetch_or(mine + (x & (kLdsWords - 1u)), 1u << (x >> 27), __ATOMIC_RELAXED,
-                                     __HIP_MEMORY_SCOPE_WORKGROUP);
+        word[j] = x & (kLdsWords - 1u);
+        bit[j] = 1u << (x >> 27);
+    }
+    uint32_t acc = 0;
+    for (uint32_t k = 0; k < iters; k += 8) {
+        const uint32_t u = ((k >> 3) * 2654435761u) >> 21;  // 11 random bits, uniform
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            acc ^= __hip_atomic_fetch_or(mine + (word[j] ^ u), bit[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
     if (acc == 0x5EED5EEDu) sink[0] = acc;  // keeps the returned values live
 }
