@@ -221,6 +221,8 @@ def main():
     ap.add_argument("--serving-count", type=int, default=600)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--extras", default="plain_n10,uneven_12_24,verify_n13,large,serving",
+                    help="comma list of the other workloads to run after the headline one")
     ap.add_argument("--tune", action="store_true",
                     help="autotune n=16 on this batch before timing (default: the committed table)")
     ap.add_argument("--copy-bytes", type=int, default=0,
@@ -341,10 +343,13 @@ def main():
         "atomics": atomics,
     }
     if not args.no_extras:
+        wanted = set(x for x in args.extras.split(",") if x)
         for name, fn in (("plain_n10", lambda: bench_plain_n10(ctx)), ("uneven_12_24", lambda: bench_uneven(ctx)),
                          ("verify_n13", lambda: bench_verify_n13(ctx)),
                          ("large", lambda: bench_large(ctx)),
                          ("serving", lambda: bench_serving(ctx, args.serving_rate, args.serving_count))):
+            if name not in wanted:
+                continue
             t0 = time.perf_counter()
             out[name] = fn()
             log(f"{name} done in {time.perf_counter() - t0:.1f} s")
