@@ -72,6 +72,14 @@ struct ssw_server {
         last_done = r->t_done;
     }
 
+    // A finished request is handed out once and then forgotten, so a
+    // long-running server holds only the requests in flight or not yet
+    // collected (the engines no longer touch a request once it is done).
+    void release(std::unordered_map<uint64_t, Request *>::iterator it) {
+        delete it->second;
+        reqs.erase(it);
+    }
+
     void fail_all(int code, const std::string &msg) {
         std::lock_guard<std::mutex> lk(mu);
         error = code;
@@ -313,6 +321,7 @@ int ssw_server_poll(ssw_server *s, uint64_t ticket, ssw_result *out) {
     if (it == s->reqs.end()) return SSW_ERR_NOTFOUND;
     if (!it->second->done) return s->error ? s->error : SSW_PENDING;
     *out = it->second->res;
+    s->release(it);
     return SSW_OK;
 }
 
@@ -329,6 +338,7 @@ int ssw_server_wait(ssw_server *s, uint64_t ticket, int32_t timeout_ms, ssw_resu
         return SSW_ERR_TIMEOUT;
     if (!r->done) return s->error;
     *out = r->res;
+    s->release(s->reqs.find(ticket));
     return SSW_OK;
 }
 

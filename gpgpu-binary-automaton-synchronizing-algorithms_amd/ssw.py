@@ -17,6 +17,14 @@ LIB_PATH = os.path.join(HERE, "libssw.so")
 
 SSW_OK = 0
 SSW_PENDING = 1
+SSW_ERR_ARG = -1
+SSW_ERR_DEVICE = -2
+SSW_ERR_NOMEM = -3
+SSW_ERR_BUDGET = -4
+SSW_ERR_STATE = -5
+SSW_ERR_INTERNAL = -6
+SSW_ERR_TIMEOUT = -7
+SSW_ERR_NOTFOUND = -8
 SSW_HOST = 0
 SSW_DEVICE = 1
 LEN_NOT_SYNC = -1

@@ -263,8 +263,12 @@ int ssw_server_create(ssw_context *ctx, const ssw_server_config *cfg, ssw_server
    request alone can never fit the budget. */
 int ssw_server_submit(ssw_server *s, int32_t n, const int32_t *a, const int32_t *b,
                       uint64_t *ticket);
-/* SSW_OK and *out filled when done, SSW_PENDING otherwise. */
+/* SSW_OK and *out filled when done, SSW_PENDING otherwise.  A result is
+   handed out once: the ticket is released by the call that returns it, and
+   later calls with it return SSW_ERR_NOTFOUND. */
 int ssw_server_poll(ssw_server *s, uint64_t ticket, ssw_result *out);
+/* As poll, blocking up to timeout_ms (< 0: no limit); SSW_ERR_TIMEOUT when
+   the result is not ready by then (the ticket stays valid). */
 int ssw_server_wait(ssw_server *s, uint64_t ticket, int32_t timeout_ms, ssw_result *out);
 int ssw_server_stats_get(ssw_server *s, ssw_server_stats *out);
 int ssw_server_destroy(ssw_server *s);

@@ -55,11 +55,16 @@ def test_admission_rejects_oversized(ctx):
         a, b = ssw.generate_cerny(30, 1)
         with pytest.raises(ssw.SswError) as e:
             srv.submit(30, a, b)
-        assert e.value.code == -4
+        assert e.value.code == ssw.SSW_ERR_BUDGET
         a, b = ssw.generate_cerny(12, 1)
-        r = srv.wait(srv.submit(12, a, b))
+        tk = srv.submit(12, a, b)
+        r = srv.wait(tk)
         assert r["length"] == 121
         assert srv.stats()["rejected"] == 1
+        # a result is handed out once; the ticket is then released
+        with pytest.raises(ssw.SswError) as e:
+            srv.poll(tk)
+        assert e.value.code == ssw.SSW_ERR_NOTFOUND
 
 
 def test_large_requests_overlap(ctx):
