@@ -14,12 +14,16 @@ Also reported on the same JSON line:
     of them; `traffic` from the committed rocprofv3 PMC summary;
   * cpu_baseline: the C restatement of the reference's SlimCPUUntweaked
     search (oracle/ssw_oracle.c) on a bounded sample, on the host cores;
+  * the LDS test-and-set rate of the step beside the device's own ceiling;
   * the other workloads, each run once: plain_n10 (10k automata, n = 10, with
-    witnesses re-verified on Q), uneven_12_24 (200k automata of uneven n in
-    one call), large (Cerny C_24..C_30 + 64 random n = 30, device-wide path)
-    and serving (Poisson stream, n in 8..30, p50/p99 latency).
+    witnesses re-verified on Q), n16_witness (the headline batch with witness
+    words, a sample re-verified on Q), uneven_12_24 (200k automata of uneven n
+    in one call), verify_n13 (Verify of 2M automata from host arrays), large
+    (Cerny C_24..C_30 + 64 random n = 30, device-wide path) and serving
+    (Poisson stream, n in 8..30, p50/p99 latency).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extras]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+                    [--no-extras | --extras name,name,...]
 """
 from __future__ import annotations
 
@@ -109,6 +113,24 @@ def bench_plain_n10(ctx):
             "automata_per_s": len(A) / dt, "ms": dt * 1e3,
             "non_synchronizing": int((r.length < 0).sum()), "max_length": int(r.length.max()),
             "witnesses_verified_on_Q": verify_words(n, A, B, r.length, r.witness)}
+
+
+def bench_n16_witness(ctx, n, delta):
+    """The headline batch with witness words (the general kernel: level sets
+    and the witness passes in HBM), host buffers, PCIe included."""
+    count = len(delta)
+    flat = np.ascontiguousarray(delta).reshape(-1)
+    ctx.solve_packed(flat[:2 * n * 1000], 1000, n, witness=True)
+    t0 = time.perf_counter()
+    r = ctx.solve_packed(flat, count, n, witness=True)
+    dt = time.perf_counter() - t0
+    idx = np.random.default_rng(19).choice(count, size=2000, replace=False)
+    A = delta[idx, :n].astype(np.int32)
+    B = delta[idx, n:].astype(np.int32)
+    return {"workload": f"the {count:,} n={n} automata above with witness words (host buffers, PCIe included)",
+            "automata_per_s": count / dt, "ms": dt * 1e3,
+            "witnesses_verified_on_Q_sample": verify_words(n, A, B, r.length[idx], r.witness[idx]),
+            "sample": 2000}
 
 
 def bench_uneven(ctx):
@@ -221,7 +243,7 @@ def main():
     ap.add_argument("--serving-count", type=int, default=600)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
-    ap.add_argument("--extras", default="plain_n10,uneven_12_24,verify_n13,large,serving",
+    ap.add_argument("--extras", default="plain_n10,n16_witness,uneven_12_24,verify_n13,large,serving",
                     help="comma list of the other workloads to run after the headline one")
     ap.add_argument("--tune", action="store_true",
                     help="autotune n=16 on this batch before timing (default: the committed table)")
@@ -344,7 +366,9 @@ def main():
     }
     if not args.no_extras:
         wanted = set(x for x in args.extras.split(",") if x)
-        for name, fn in (("plain_n10", lambda: bench_plain_n10(ctx)), ("uneven_12_24", lambda: bench_uneven(ctx)),
+        for name, fn in (("plain_n10", lambda: bench_plain_n10(ctx)),
+                         ("n16_witness", lambda: bench_n16_witness(ctx, n, delta)),
+                         ("uneven_12_24", lambda: bench_uneven(ctx)),
                          ("verify_n13", lambda: bench_verify_n13(ctx)),
                          ("large", lambda: bench_large(ctx)),
                          ("serving", lambda: bench_serving(ctx, args.serving_rate, args.serving_count))):
