@@ -762,8 +762,9 @@ struct GroupSolver {
 };
 
 // Small automata (n <= 16) without witness: the whole live frontier sits in an
-// LDS ring of 16-bit subsets (ring_cap entries; the largest two-level window
-// of C_16 is 3099 subsets), images come from packed tables (a | b << 16) of
+// LDS ring of 16-bit subsets (ring_cap entries; the unread window is at most
+// the wider of two consecutive levels, and C_16's widest level holds 1577
+// subsets), images come from packed tables (a | b << 16) of
 // four nibble slices (BT = false, 256 bytes) or two byte slices (BT = true,
 // 2 KiB, half the lookups), and every lane takes up to T frontier entries per
 // level step.  A frontier that outgrows the ring is either listed for a
