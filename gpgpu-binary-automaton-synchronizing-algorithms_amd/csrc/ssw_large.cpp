@@ -333,13 +333,24 @@ int LargeEngine::solve_indices(ssw_context &ctx, const DeviceBatch &db, const ui
             if (!slots[s].used) continue;
             const LargeInstance &I = hinst[s];
             if (I.status == kLError) {
+                // read before the slot is cleared
                 const int64_t bad = I.index;
+                const int32_t why = I.length;
                 retire(s);
-                clear_slot(ctx, s);
-                if (I.length == -3) {
+                rc = clear_slot(ctx, s);
+                if (rc) return rc;
+                if (why == -3) {
                     // invalid transitions: report like the batched kernel
+                    // (length SSW_LEN_INVALID, flag 0, nothing expanded)
                     static const int32_t v = -3;
+                    static const uint8_t no_sync = 0;
+                    static const int64_t none = 0;
                     LG_TRY(hipMemcpyAsync(db.length + bad, &v, 4, hipMemcpyHostToDevice, ctx.stream), "H2D");
+                    if (db.sync)
+                        LG_TRY(hipMemcpyAsync(db.sync + bad, &no_sync, 1, hipMemcpyHostToDevice, ctx.stream), "H2D");
+                    if (db.expanded)
+                        LG_TRY(hipMemcpyAsync(db.expanded + bad, &none, 8, hipMemcpyHostToDevice, ctx.stream),
+                               "H2D");
                     if (append)
                         LG_TRY(hipMemsetAsync(db.witness + (uint64_t)bad * db.witness_stride, 0, db.witness_stride,
                                               ctx.stream),

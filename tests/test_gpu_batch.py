@@ -171,3 +171,47 @@ def test_host_arrays_bad_transition_in_last_chunk(ctx):
     r = ctx.compute(n, A[:1000], B[:1000])
     L, _, _, _ = O.slim(n, A[:1000], B[:1000])
     np.testing.assert_array_equal(r.length, L)
+
+
+def test_empty_batches(ctx):
+    A = np.zeros((0, 7), np.int32)
+    r = ctx.compute(7, A, A, witness=True)
+    assert r.length.shape == (0,) and r.violation == -1
+    assert ctx.verify(7, A, A) == -1
+    for mode in ("batch", "plain", "large"):
+        r = ctx.solve_packed(np.zeros(1, np.uint8), 0, 7, mode=mode)
+        assert r.length.shape == (0,) and r.violation == -1
+
+
+@pytest.mark.parametrize("mode", ["batch", "plain", "large"])
+def test_device_batch_marks_invalid_automata(ctx, mode):
+    # host batches are checked on the host (the call fails); a device batch is
+    # checked by the kernels: an automaton with a transition >= n is skipped
+    # with length SSW_LEN_INVALID and the others are solved
+    n = 9
+    A, B = ssw.generate_uniform(5, n, 64)
+    A[3, 2] = n
+    B[40, 8] = 200
+    delta = ssw.pack_uniform(A, B).reshape(-1)
+    with pytest.raises(ssw.SswError):
+        ctx.solve_packed(delta, 64, n, mode=mode)
+    count = 64
+    d = ctx.to_device(delta)
+    dl, ds, de = ctx.alloc(4 * count), ctx.alloc(count), ctx.alloc(8 * count)
+    try:
+        b = ssw.Batch(count=count, uniform_n=n, witness_stride=0, n_per=None, offset=None, delta=d, length=dl,
+                      sync=ds, expanded=de, witness=None, bound=-1, reserved=0)
+        ctx.solve_device(b, mode=mode)
+        L = ctx.to_host(dl, np.int32, count)
+        S = ctx.to_host(ds, np.uint8, count)
+        E = ctx.to_host(de, np.int64, count)
+    finally:
+        for p in (d, dl, ds, de):
+            ctx.free(p)
+    bad = [3, 40]
+    assert np.all(L[bad] == ssw.LEN_INVALID) and np.all(S[bad] == 0) and np.all(E[bad] == 0)
+    good = np.setdiff1d(np.arange(count), bad)
+    Lo, So, Eo, _ = O.slim(n, A[good], B[good])
+    np.testing.assert_array_equal(L[good], Lo)
+    np.testing.assert_array_equal(S[good], So)
+    np.testing.assert_array_equal(E[good], Eo)
