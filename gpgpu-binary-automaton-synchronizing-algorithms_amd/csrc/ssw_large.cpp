@@ -213,14 +213,19 @@ int LargeEngine::read_back(ssw_context &ctx) {
         LargeSlot &S = slots[i];
         LargeInstance &I = hinst[i];
         if (!S.used || I.status != kLPause) continue;
-        // grow the ring (or the level table) so the next level fits
+        // grow the ring (or the level table) so the next level fits; the
+        // same bound as large_plan's: at most 2 * width new subsets, and no
+        // more than the 2^n distinct ones in the ring
         const uint64_t width = I.hi - I.lo;
-        const uint64_t need = I.append ? I.hi + 2 * width : 3 * width;
+        const uint64_t all = 1ull << S.n;
+        const uint64_t need = std::min<uint64_t>(I.append ? I.hi + 2 * width : 3 * width, all);
         if (need > S.ring_cap) {
-            const uint64_t all = 1ull << S.n;
             uint64_t cap = pow2_at_least(std::max<uint64_t>(2 * need, 2 * S.ring_cap));
             if (!I.append) cap = std::min<uint64_t>(cap, pow2_at_least(all));
             else cap = std::min<uint64_t>(cap, pow2_at_least(all + 1));
+            if (cap < need)
+                return ctx.fail(SSW_ERR_INTERNAL, "device-wide path: ring of %llu entries for a level needing %llu",
+                                (unsigned long long)cap, (unsigned long long)need);
             uint32_t *nr = nullptr;
             LG_TRY(hipMalloc((void **)&nr, 4 * cap), "grow ring");
             const uint64_t from = I.append ? 0 : I.lo;

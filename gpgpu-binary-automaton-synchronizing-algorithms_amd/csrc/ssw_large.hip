@@ -133,8 +133,11 @@ __global__ __launch_bounds__(256) void large_plan(LargeState *st, LargeInstance 
             }
             if (I.status == kLRun) {
                 const uint64_t width = I.hi - I.lo;
-                // the next level holds at most 2 * width subsets
-                const uint64_t need = I.append ? I.hi + 2 * width : 3 * width;
+                // the next level holds at most 2 * width subsets, and the
+                // ring never holds more than the 2^n distinct subsets
+                const uint64_t bound = I.append ? I.hi + 2 * width : 3 * width;
+                const uint64_t all = 1ull << I.n;
+                const uint64_t need = bound < all ? bound : all;
                 if (need > I.ring_mask + 1 || (I.append && (uint32_t)I.level + 2 >= I.lvl_cap))
                     I.status = kLPause;
                 else

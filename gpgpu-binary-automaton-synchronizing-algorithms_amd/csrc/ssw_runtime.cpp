@@ -800,6 +800,11 @@ int ssw_context::solve_entry(const ssw_batch *b, int32_t memory, int mode, int64
                 solving.store(false);
                 return hip_fail(e, "D2H n_per");
             }
+            for (int64_t i = 0; i < b->count; ++i)
+                if (host_n[i] < 1 || host_n[i] > SSW_MAX_N) {
+                    solving.store(false);
+                    return fail(SSW_ERR_ARG, "automaton %lld: n=%d out of [1,30]", (long long)i, (int)host_n[i]);
+                }
         }
         const uint8_t *hn = b->uniform_n > 0 ? nullptr : host_n.data();
         rc = mode == kModeLarge ? large.solve_all(*this, db, hn, violation_out)

@@ -215,3 +215,43 @@ def test_device_batch_marks_invalid_automata(ctx, mode):
     np.testing.assert_array_equal(L[good], Lo)
     np.testing.assert_array_equal(S[good], So)
     np.testing.assert_array_equal(E[good], Eo)
+
+
+@pytest.mark.parametrize("witness", [True, False])
+@pytest.mark.parametrize("mode", ["batch", "plain", "large"])
+def test_tiny_n_every_mode(ctx, mode, witness):
+    # n = 1 (Q is a singleton: not synchronizing by the reference's rule, E = 1)
+    # up to n = 5, in every entry point.  For such n a level's 2 * width images
+    # can outnumber the 2^n subsets: the device-wide ring is sized to the
+    # subsets, not to that bound.
+    for n in (1, 2, 3, 4, 5):
+        A, B = ssw.generate_uniform(11 + n, n, 200)
+        if n >= 2:
+            A[0], B[0] = ssw.generate_cerny(n, 1)
+        r = ctx.solve_uniform(n, A, B, witness=witness, mode=mode)
+        L, S, E, _ = O.slim(n, A, B)
+        np.testing.assert_array_equal(r.length, L, err_msg=f"n={n}")
+        np.testing.assert_array_equal(r.sync, S, err_msg=f"n={n}")
+        np.testing.assert_array_equal(r.expanded, E, err_msg=f"n={n}")
+        if witness:
+            for i in np.nonzero(L >= 0)[0]:
+                w = ssw.witness_bits_to_word(r.witness[i], int(L[i]))
+                np.testing.assert_array_equal(w, O.witness(n, A[i], B[i])[1], err_msg=f"n={n} i={i}")
+
+
+@pytest.mark.parametrize("mode", ["batch", "plain", "large"])
+def test_device_batch_with_bad_size_rejected(ctx, mode):
+    n_per = np.array([5, 0, 7], np.uint8)
+    offs = np.array([0, 10, 10], np.uint64)
+    delta = np.zeros(24, np.uint8)
+    d = [ctx.to_device(x) for x in (delta, n_per, offs)]
+    dl = ctx.alloc(12)
+    try:
+        b = ssw.Batch(count=3, uniform_n=0, witness_stride=0, n_per=d[1], offset=d[2], delta=d[0], length=dl,
+                      sync=None, expanded=None, witness=None, bound=-1, reserved=0)
+        with pytest.raises(ssw.SswError) as e:
+            ctx.solve_device(b, mode=mode)
+        assert e.value.code == ssw.SSW_ERR_ARG
+    finally:
+        for p in d + [dl]:
+            ctx.free(p)
