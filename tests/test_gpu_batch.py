@@ -255,3 +255,67 @@ def test_device_batch_with_bad_size_rejected(ctx, mode):
     finally:
         for p in d + [dl]:
             ctx.free(p)
+
+
+@pytest.mark.parametrize("mode", ["batch", "plain", "large"])
+def test_violation_index_every_mode(ctx, mode):
+    # a lowered bound: the first automaton whose length exceeds it
+    # (SlimCPUUntweaked.cs:189-213), from every entry point
+    n = 11
+    A, B = ssw.generate_uniform(808, n, 300)
+    L, _, _, v = O.slim(n, A, B, bound=8)
+    assert v >= 0
+    r = ctx.solve_uniform(n, A, B, bound=8, mode=mode)
+    assert r.violation == v
+    np.testing.assert_array_equal(r.length, L)
+
+
+def test_offsets_in_any_order(ctx):
+    # offsets need not follow the automaton order: the batch packed backwards
+    rng = np.random.default_rng(21)
+    ns = rng.integers(2, 17, size=500).astype(np.uint8)
+    As, Bs = [], []
+    for i, n in enumerate(ns):
+        a, b = ssw.generate_uniform(99, int(n), 1, first=i)
+        As.append(a[0])
+        Bs.append(b[0])
+    n_per, offs, delta = ssw.pack(ns, As, Bs)
+    # rebuild delta with the automata stored in reverse order
+    sizes = 2 * ns.astype(np.uint64)
+    roffs = np.zeros(len(ns), np.uint64)
+    pos = 0
+    rdelta = np.empty_like(delta)
+    for i in reversed(range(len(ns))):
+        roffs[i] = pos
+        rdelta[pos:pos + int(sizes[i])] = delta[int(offs[i]):int(offs[i]) + int(sizes[i])]
+        pos += int(sizes[i])
+    r1 = ctx.solve_packed(delta, len(ns), 0, n_per, offs, witness=True)
+    r2 = ctx.solve_packed(rdelta, len(ns), 0, n_per, roffs, witness=True)
+    np.testing.assert_array_equal(r1.length, r2.length)
+    np.testing.assert_array_equal(r1.expanded, r2.expanded)
+    np.testing.assert_array_equal(r1.witness, r2.witness)
+
+
+@pytest.mark.parametrize("mode", ["batch", "plain", "large"])
+def test_wide_witness_rows_zero_padded(ctx, mode):
+    # rows wider than the word: the bytes past it are zero
+    n = 10
+    A, B = ssw.generate_uniform(31, n, 50)
+    stride = 40
+    delta = ssw.pack_uniform(A, B).reshape(-1)
+    length = np.empty(50, np.int32)
+    wit = np.full((50, stride), 0xAB, np.uint8)
+    b = ssw.Batch(count=50, uniform_n=n, witness_stride=stride, n_per=None, offset=None,
+                  delta=delta.ctypes.data, length=length.ctypes.data, sync=None, expanded=None,
+                  witness=wit.ctypes.data, bound=-1, reserved=0)
+    fn = {"batch": ctx._L.ssw_solve_batch, "plain": ctx._L.ssw_solve_plain, "large": ctx._L.ssw_solve_large}[mode]
+    viol = ssw.ctypes.c_int64(-1)
+    assert fn(ctx.h, ssw.ctypes.byref(b), ssw.SSW_HOST, ssw.ctypes.byref(viol)) == 0
+    L, _, _, _ = O.slim(n, A, B)
+    np.testing.assert_array_equal(length, L)
+    for i in range(50):
+        used = (int(L[i]) + 7) // 8 if L[i] > 0 else 0
+        assert np.all(wit[i, used:] == 0), (i, L[i])
+        if L[i] > 0:
+            w = ssw.witness_bits_to_word(wit[i], int(L[i]))
+            np.testing.assert_array_equal(w, O.witness(n, A[i], B[i])[1])

@@ -82,3 +82,21 @@ def test_large_requests_overlap(ctx):
     for (n, a, b), r in zip(reqs, res):
         L, S, E, _ = O.solve(n, np.asarray(a)[None], np.asarray(b)[None])
         assert (r["length"], r["expanded"]) == (L[0], E[0])
+
+
+@pytest.mark.parametrize("want_witness", [False, True])
+def test_everything_in_the_device_wide_engine(ctx, want_witness):
+    # small_n_max = 0 routes every request, n = 1 included, to the device-wide
+    # engine; results equal the oracle's
+    with ssw.Server(ctx, small_n_max=0, want_witness=want_witness) as srv:
+        reqs = []
+        for n in range(1, 9):
+            A, B = ssw.generate_uniform(600 + n, n, 6)
+            for i in range(len(A)):
+                reqs.append((n, A[i], B[i], srv.submit(n, A[i], B[i])))
+        for n, a, b, tk in reqs:
+            r = srv.wait(tk, 60000)
+            L, S, E, _ = O.slim(n, a[None], b[None])
+            assert r["length"] == L[0] and r["expanded"] == E[0] and r["sync"] == bool(S[0]), (n, r)
+            if want_witness and L[0] >= 0:
+                np.testing.assert_array_equal(r["witness"], O.witness(n, a, b)[1])
