@@ -319,3 +319,16 @@ def test_wide_witness_rows_zero_padded(ctx, mode):
         if L[i] > 0:
             w = ssw.witness_bits_to_word(wit[i], int(L[i]))
             np.testing.assert_array_equal(w, O.witness(n, A[i], B[i])[1])
+
+
+def test_host_arrays_chunked_n20(ctx):
+    # the chunked host path for n > 16 (HBM bitmaps, hand-off of long
+    # frontiers to the device-wide path) against the oracle
+    n = 20
+    count = 270_000
+    A, B = ssw.generate_uniform(2020, n, count)
+    L, S, E, _ = O.solve(n, A, B, threads=8)
+    r = ctx.compute(n, A, B)
+    np.testing.assert_array_equal(r.length, L)
+    np.testing.assert_array_equal(r.sync, S)
+    np.testing.assert_array_equal(r.expanded, E)

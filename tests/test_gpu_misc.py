@@ -80,3 +80,32 @@ def test_contexts_on_threads():
     for t in ts:
         t.join()
     assert not errs
+
+
+def test_autotune_on_a_caller_batch(ctx):
+    # ssw_autotune_batch times the candidate configurations on the caller's
+    # device batch (here n = 14 with Cerny stragglers) and keeps the fastest;
+    # results stay exact under the chosen entry
+    saved = ctx.get_tuning(14)
+    n, delta, pos = W.uniform_n16(seed=4242, count=20_000, n=14, cerny_every=200)
+    count = len(delta)
+    d = ctx.to_device(delta.reshape(-1))
+    dl, de = ctx.alloc(4 * count), ctx.alloc(8 * count)
+    try:
+        b = ssw.Batch(count=count, uniform_n=n, witness_stride=0, n_per=None, offset=None, delta=d, length=dl,
+                      sync=None, expanded=de, witness=None, bound=-1, reserved=0)
+        ctx.autotune_batch(b)
+        ctx.solve_device(b)
+        L = ctx.to_host(dl, np.int32, count)
+        E = ctx.to_host(de, np.int64, count)
+    finally:
+        for p in (d, dl, de):
+            ctx.free(p)
+        chosen = ctx.get_tuning(14)
+        ctx.set_tuning(14, **saved)
+    assert chosen["group_lanes"] in (16, 32, 64, 256)
+    assert np.all(L[pos] == (n - 1) ** 2) and np.all(E[pos] == 2 ** n - n - 1)
+    idx = np.union1d(np.arange(0, count, 37), pos[:5])
+    Lo, _, Eo, _ = O.slim(n, delta[idx, :n].astype(np.int32), delta[idx, n:].astype(np.int32))
+    np.testing.assert_array_equal(L[idx], Lo)
+    np.testing.assert_array_equal(E[idx], Eo)
