@@ -85,3 +85,35 @@ def test_batch_defers_with_witness(ctx):
     for i in range(len(A)):
         if L[i] >= 0:
             assert O.verify_word(n, A[i], B[i], ssw.witness_bits_to_word(r.witness[i], int(L[i])))
+
+
+def _two_closed_halves(h, seed):
+    # two closed halves of h states each (C_h on one, a random automaton on
+    # the other): never synchronizing, and the pairs of subsets one word
+    # reaches in the two halves make a wide reachable family
+    a1, b1 = ssw.generate_cerny(h, 1)
+    A2, B2 = ssw.generate_uniform(seed, h, 1)
+    a = np.concatenate([a1, A2[0] + h]).astype(np.int32)
+    b = np.concatenate([b1, B2[0] + h]).astype(np.int32)
+    return a, b
+
+
+@pytest.mark.parametrize("mode", ["batch", "plain", "large"])
+@pytest.mark.parametrize("witness", [False, True])
+@pytest.mark.parametrize("h,seed,least", [(8, 7, 30_000), (10, 36, 100_000)])
+def test_non_synchronizing_with_a_wide_family(ctx, mode, witness, h, seed, least):
+    # n = 16 runs in the small-automaton kernels (ring spill chain), n = 20
+    # in the HBM-bitmap ones and the device-wide path
+    n = 2 * h
+    a, b = _two_closed_halves(h, seed)
+    c, d = ssw.generate_cerny(n, 1)
+    A = np.stack([a, c, a])
+    B = np.stack([b, d, b])
+    L, S, E, _ = O.solve(n, A, B, threads=4)
+    assert L[0] == -1 and E[0] > least
+    r = ctx.solve_uniform(n, A, B, witness=witness, mode=mode)
+    np.testing.assert_array_equal(r.length, L)
+    np.testing.assert_array_equal(r.sync, S)
+    np.testing.assert_array_equal(r.expanded, E)
+    if witness:
+        assert not r.witness[0].any() and not r.witness[2].any()
