@@ -117,3 +117,16 @@ def test_non_synchronizing_with_a_wide_family(ctx, mode, witness, h, seed, least
     np.testing.assert_array_equal(r.expanded, E)
     if witness:
         assert not r.witness[0].any() and not r.witness[2].any()
+
+
+@pytest.mark.parametrize("n", [25, 26])
+def test_large_cerny_witness_beyond_the_oracle(ctx, n):
+    # n > 24 is past the oracle's witness restatement: the closed forms pin it
+    # (length (n-1)^2, E = 2^n - n - 1, the word (b a^(n-1))^(n-2) b)
+    a, b = ssw.generate_cerny(n, 1)
+    r = ctx.solve_uniform(n, a[None], b[None], witness=True, mode="large")
+    assert r.length[0] == (n - 1) ** 2
+    assert r.expanded[0] == 2 ** n - n - 1
+    w = ssw.witness_bits_to_word(r.witness[0], (n - 1) ** 2)
+    np.testing.assert_array_equal(w, _cerny_word(n, 1))
+    assert O.verify_word(n, a, b, w)

@@ -100,3 +100,19 @@ def test_everything_in_the_device_wide_engine(ctx, want_witness):
             assert r["length"] == L[0] and r["expanded"] == E[0] and r["sync"] == bool(S[0]), (n, r)
             if want_witness and L[0] >= 0:
                 np.testing.assert_array_equal(r["witness"], O.witness(n, a, b)[1])
+
+
+def test_destroy_with_requests_in_flight(ctx):
+    # destroying the server while requests are queued or running drains them
+    # and returns; nothing leaks into the next server
+    srv = ssw.Server(ctx, small_n_max=16)
+    for req in _stream(200, 24, seed=11):
+        _, n, a, b, _ = req
+        srv.submit(n, a, b)
+    t0 = time.perf_counter()
+    srv.close()
+    assert time.perf_counter() - t0 < 60
+    with ssw.Server(ctx, small_n_max=16) as srv2:
+        a, b = ssw.generate_cerny(9, 1)
+        assert srv2.wait(srv2.submit(9, a, b))["length"] == 64
+        assert srv2.stats()["completed"] == 1
