@@ -109,3 +109,51 @@ def test_autotune_on_a_caller_batch(ctx):
     Lo, _, Eo, _ = O.slim(n, delta[idx, :n].astype(np.int32), delta[idx, n:].astype(np.int32))
     np.testing.assert_array_equal(L[idx], Lo)
     np.testing.assert_array_equal(E[idx], Eo)
+
+
+def test_one_context_shared_by_threads(ctx):
+    # calls on one context from several threads are serialised by the library
+    A, B = ssw.generate_uniform(3, 13, 30_000)
+    want = O.slim(13, A, B, threads=8)[0]
+    errs = []
+
+    def work(k):
+        try:
+            for _ in range(3):
+                r = ctx.compute(13, A[k::4], B[k::4])
+                if not np.array_equal(r.length, want[k::4]):
+                    errs.append(k)
+        except Exception as e:  # noqa: BLE001
+            errs.append(repr(e))
+
+    ts = [threading.Thread(target=work, args=(k,)) for k in range(4)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    assert not errs
+
+
+def test_small_budget_rejects_what_cannot_fit():
+    # a context with a 32 MiB budget cannot hold C_28's 32 MiB bitmap plus its
+    # ring: the device-wide path says so instead of failing an allocation
+    with ssw.Context(0, budget=32 << 20) as c:
+        a, b = ssw.generate_cerny(28, 1)
+        with pytest.raises(ssw.SswError) as e:
+            c.solve_uniform(28, a[None], b[None], mode="large")
+        assert e.value.code == ssw.SSW_ERR_BUDGET
+        # small automata still solve under the same budget
+        A, B = ssw.generate_uniform(5, 12, 2000)
+        np.testing.assert_array_equal(c.compute(12, A, B).length, O.slim(12, A, B)[0])
+
+
+def test_verify_bound_zero():
+    # with bound 0 every synchronizing automaton violates: Verify returns the
+    # first one (IComputable.Verify, SlimCPUUntweaked.cs:189-213,225-226)
+    with ssw.Context(0) as c:
+        A, B = ssw.generate_uniform(8, 9, 500)
+        A[:3] = np.arange(9)  # a permutation...
+        B[:3] = np.arange(9)  # ... and the identity: not synchronizing
+        L, _, _, v = O.slim(9, A, B, bound=0)
+        assert v == int(np.nonzero(L >= 1)[0][0]) >= 3
+        assert c.verify(9, A, B, bound=0) == v
