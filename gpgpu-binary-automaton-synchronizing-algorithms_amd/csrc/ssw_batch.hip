@@ -417,19 +417,37 @@ __device__ uint64_t BlockGroup<NT>::live_segments(const uint32_t *queues) const 
 // Sets up the segments of a batch of `count` work items, or (empty = 1) the
 // same segments with no items yet, for a launch whose items are appended by
 // the one before it.
-__global__ void init_queues_kernel(uint32_t *queues, int64_t count, uint32_t empty) {
+// Sets up to three work queues over `count` items in one launch: queue j
+// (non-null) gets the 64 segment boundaries, full or (bit j of empty_mask)
+// empty, and its exhausted mask.
+__global__ void init_queues_kernel(uint32_t *q0, uint32_t *q1, uint32_t *q2, int64_t count, uint32_t empty_mask) {
     const int k = threadIdx.x;
-    if (k < kQueues) {
-        const uint32_t s = (uint32_t)(count * k / kQueues), e = (uint32_t)(count * (k + 1) / kQueues);
-        queues[k * kQueueWords] = s;
-        queues[k * kQueueWords + 1] = empty ? s : e;
+    uint32_t *qs[3] = {q0, q1, q2};
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        uint32_t *queues = qs[j];
+        if (!queues) continue;
+        if (k < kQueues) {
+            const uint32_t s = (uint32_t)(count * k / kQueues), e = (uint32_t)(count * (k + 1) / kQueues);
+            queues[k * kQueueWords] = s;
+            queues[k * kQueueWords + 1] = ((empty_mask >> j) & 1u) ? s : e;
+        }
     }
     __syncthreads();
-    if (k == 0) {
+    if (k < 3 && qs[k]) {
+        uint32_t *queues = qs[k];
         unsigned long long m = 0;
         for (int j = 0; j < kQueues; ++j)
             if (queues[j * kQueueWords] >= queues[j * kQueueWords + 1]) m |= 1ull << j;
         *(unsigned long long *)(queues + kQueues * kQueueWords) = m;
+    }
+}
+
+// The per-solve counters: deferred count <- 0, smallest violating index <- none.
+__global__ void reset_counters_kernel(uint32_t *deferred, unsigned long long *violation) {
+    if (threadIdx.x == 0) {
+        *deferred = 0u;
+        *violation = ~0ull;
     }
 }
 
@@ -1506,7 +1524,17 @@ size_t batch_block_lds(int variant, int group_lanes, int groups, uint32_t bitmap
 }
 
 hipError_t init_queues(uint32_t *queues, int64_t count, hipStream_t st, bool empty) {
-    hipLaunchKernelGGL(init_queues_kernel, dim3(1), dim3(kQueues), 0, st, queues, count, empty ? 1u : 0u);
+    return init_queues3(queues, nullptr, nullptr, count, empty ? 1u : 0u, st);
+}
+
+hipError_t init_queues3(uint32_t *q0, uint32_t *q1, uint32_t *q2, int64_t count, uint32_t empty_mask,
+                        hipStream_t st) {
+    hipLaunchKernelGGL(init_queues_kernel, dim3(1), dim3(kQueues), 0, st, q0, q1, q2, count, empty_mask);
+    return hipGetLastError();
+}
+
+hipError_t reset_counters(uint32_t *deferred, unsigned long long *violation, hipStream_t st) {
+    hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, st, deferred, violation);
     return hipGetLastError();
 }
 

@@ -75,6 +75,7 @@ struct PassIO {
     int narrow_bits = 0;                // relabelled states, 2^narrow_bits-bit bitmap
     bool byte_tab = false;
     bool force_small = false;           // the n <= 16 solver on a class of larger n (relabelled)
+    bool queue_ready = false;           // the class's queue was set up with the spill queues
 };
 constexpr int kClassQueues = 3;  // work queues per launch class: its own and two spill queues
 

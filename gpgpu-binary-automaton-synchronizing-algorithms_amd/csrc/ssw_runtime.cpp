@@ -31,6 +31,9 @@ size_t batch_group_lds(int variant, uint32_t bitmap_words, uint32_t arena_cap);
 size_t batch_block_lds(int variant, int group_lanes, int groups, uint32_t bitmap_words, uint32_t arena_cap);
 hipError_t fill_u32(uint32_t *p, uint64_t words, uint32_t value, hipStream_t st);
 hipError_t init_queues(uint32_t *queues, int64_t count, hipStream_t st, bool empty);
+hipError_t init_queues3(uint32_t *q0, uint32_t *q1, uint32_t *q2, int64_t count, uint32_t empty_mask,
+                        hipStream_t st);
+hipError_t reset_counters(uint32_t *deferred, unsigned long long *violation, hipStream_t st);
 }  // namespace ssw
 
 using namespace ssw;
@@ -243,10 +246,12 @@ int ssw_context::launch_class(const DeviceBatch &db, const ClassSpec &cls, bool 
         return launch_pass(db, cls, t, plain, PassIO{});
     const int nb = (t.narrow_bits >= 8 && t.narrow_bits < cls.nmax) ? t.narrow_bits : 0;
     hipStream_t st = cls.stream ? cls.stream : stream;
-    // the later launches' queues: the first launch's segments, empty
-    for (int j = 0; j < (nb ? 2 : 1); ++j)
-        HIP_TRY(init_queues(cls.spill_queue[j], cls.items, st, true), "init queues");
+    // one launch sets up the class's queue (full) and the later launches'
+    // (the first launch's segments, empty)
+    HIP_TRY(init_queues3(cls.queues, cls.spill_queue[0], nb ? cls.spill_queue[1] : nullptr, cls.items, 6u, st),
+            "init queues");
     PassIO first;
+    first.queue_ready = true;
     first.kind = kPassSpillFrom;
     first.out_list = cls.spill_list[0];
     first.out_queue = cls.spill_queue[0];
@@ -441,7 +446,7 @@ int ssw_context::launch_pass(const DeviceBatch &db, const ClassSpec &cls, const 
     last_variant = variant;
 
     if (!plain) {
-        if (!io.in_queue) HIP_TRY(init_queues(P.queues, P.count, st, false), "init queues");
+        if (!io.in_queue && !io.queue_ready) HIP_TRY(init_queues(P.queues, P.count, st, false), "init queues");
         HIP_TRY(launch_batch(variant, G, P, grid, block, lds, st), "batch kernel launch");
         ++launches;
     } else {
@@ -465,8 +470,9 @@ int ssw_context::solve_device(const DeviceBatch &db, const uint8_t *host_n, bool
     int rc = ensure(deferred, 4ull * std::max<int64_t>(1, db.count), "deferred list");
     if (rc) return rc;
     // violation <- UINT64_MAX, deferred count <- 0
-    HIP_TRY(fill_u32((uint32_t *)((uint8_t *)ctrs.p + kCtrViolation), 2, 0xFFFFFFFFu, stream), "fill");
-    HIP_TRY(fill_u32((uint32_t *)((uint8_t *)ctrs.p + kCtrDeferred), 1, 0, stream), "fill");
+    HIP_TRY(reset_counters((uint32_t *)((uint8_t *)ctrs.p + kCtrDeferred),
+                           (unsigned long long *)((uint8_t *)ctrs.p + kCtrViolation), stream),
+            "reset counters");
     HIP_TRY(hipEventRecord(ev0, stream), "hipEventRecord");
 
     int nmin = db.uniform_n, nmaxall = db.uniform_n;
