@@ -1523,14 +1523,14 @@ size_t batch_block_lds(int variant, int group_lanes, int groups, uint32_t bitmap
                                  (uint32_t)scratch);
 }
 
-hipError_t init_queues(uint32_t *queues, int64_t count, hipStream_t st, bool empty) {
-    return init_queues3(queues, nullptr, nullptr, count, empty ? 1u : 0u, st);
-}
-
 hipError_t init_queues3(uint32_t *q0, uint32_t *q1, uint32_t *q2, int64_t count, uint32_t empty_mask,
                         hipStream_t st) {
     hipLaunchKernelGGL(init_queues_kernel, dim3(1), dim3(kQueues), 0, st, q0, q1, q2, count, empty_mask);
     return hipGetLastError();
+}
+
+hipError_t init_queues(uint32_t *queues, int64_t count, hipStream_t st, bool empty) {
+    return init_queues3(queues, nullptr, nullptr, count, empty ? 1u : 0u, st);
 }
 
 hipError_t reset_counters(uint32_t *deferred, unsigned long long *violation, hipStream_t st) {
