@@ -157,3 +157,45 @@ def test_verify_bound_zero():
         L, _, _, v = O.slim(9, A, B, bound=0)
         assert v == int(np.nonzero(L >= 1)[0][0]) >= 3
         assert c.verify(9, A, B, bound=0) == v
+
+
+def test_tuning_validation_and_malformed_tables(ctx, tmp_path):
+    saved = ctx.get_tuning(12)
+    bad = [dict(group_lanes=48, block_threads=128, chunk=4, arena_lds=512),
+           dict(group_lanes=64, block_threads=96, chunk=4, arena_lds=512),
+           dict(group_lanes=64, block_threads=128, chunk=0, arena_lds=512),
+           dict(group_lanes=64, block_threads=128, chunk=4, arena_lds=512, lane_entries=3),
+           dict(group_lanes=64, block_threads=128, chunk=4, arena_lds=512, narrow_bits=20)]
+    for t in bad:
+        with pytest.raises(ssw.SswError) as e:
+            ctx.set_tuning(12, **t)
+        assert e.value.code == ssw.SSW_ERR_ARG, t
+    assert ctx.get_tuning(12) == saved
+    # a table that cannot be read or holds no tuning object is refused
+    p = tmp_path / "broken.json"
+    p.write_text("{ not json")
+    with pytest.raises(ssw.SswError):
+        ctx.load_tuning(str(p))
+    with pytest.raises(ssw.SswError):
+        ctx.load_tuning(str(tmp_path / "missing.json"))
+    assert ctx.get_tuning(12) == saved
+    A, B = ssw.generate_uniform(9, 12, 3000)
+    np.testing.assert_array_equal(ctx.compute(12, A, B).length, O.slim(12, A, B)[0])
+
+
+def test_server_tickets_and_timeouts(ctx):
+    with ssw.Server(ctx, small_n_max=16) as srv:
+        with pytest.raises(ssw.SswError) as e:
+            srv.poll(123456789)
+        assert e.value.code == ssw.SSW_ERR_NOTFOUND
+        a, b = ssw.generate_cerny(30, 1)  # about 0.14 s of work
+        tk = srv.submit(30, a, b)
+        with pytest.raises(ssw.SswError) as e:
+            srv.wait(tk, 1)
+        assert e.value.code == ssw.SSW_ERR_TIMEOUT
+        r = srv.wait(tk, 120000)  # the ticket survives a timed-out wait
+        assert r["length"] == 29 * 29
+        with pytest.raises(ssw.SswError):
+            srv.submit(31, np.zeros(31, np.int32), np.zeros(31, np.int32))
+        with pytest.raises(ssw.SswError):
+            srv.submit(5, np.array([0, 1, 2, 3, 5], np.int32), np.zeros(5, np.int32))
