@@ -796,8 +796,16 @@ struct GroupSolver {
 // a bitmap word or table entry address is one OR of the base and the scaled
 // index; lanes without a new entry write a dummy ring entry instead of
 // branching around the write.
-template <class Grp, int T, bool BT>
+// WIT (one wavefront per automaton only): the search also records, in the
+// group's HBM slot, every discovered subset in discovery order (hist) and the
+// start of every level in it (lvlp); a found automaton then gets the
+// lexicographically smallest shortest reset word from a backward pass over
+// the levels (good marks of the next level in the LDS bitmap, which the
+// search no longer needs, and one flag per recorded position in HBM) and a
+// forward greedy pass.
+template <class Grp, int T, bool BT, bool WIT = false>
 struct SmallSolver {
+    static_assert(!WIT || Grp::kSize == 64, "witness words: one wavefront per automaton");
     static constexpr uint32_t G = Grp::kSize;
     static constexpr uint32_t kGlobalRing = 65536;
     // one persistent loop per automaton when a group is a whole wavefront or
@@ -815,6 +823,9 @@ struct SmallSolver {
     static constexpr uint32_t kDummies = G == 64 ? 64u : (G >= 32 ? 32u : 16u);
     uint32_t dum_a;
     glb_u16 *gring;
+    glb_u16 *hist;   // WIT: subsets in discovery order, 65536 entries
+    glb_u8 *goodp;   // WIT: 1 = the subset at this position of hist is on a shortest path
+    glb_u32 *lvlp;   // WIT: start position of each level in hist
     uint32_t rmask, rcap;
     bool in_lds;
     bool narrow;  // relabelled states, visited bitmap of 2^narrow_bits bits
@@ -848,6 +859,9 @@ struct SmallSolver {
         ctl_a = pimg_a + 64;
         ring_a = ctl_a + 128;
         gring = (glb_u16 *)(slot + P.garena_off);
+        hist = (glb_u16 *)(slot + P.hist_off);
+        goodp = (glb_u8 *)(slot + P.good_off);
+        lvlp = (glb_u32 *)(slot + P.lvl_off);
         rcap = P.lds_arena_cap;
         rmask = rcap - 1u;
         dum_a = stage_a + 4u * (g.rank() & (kDummies - 1u));
@@ -947,6 +961,7 @@ struct SmallSolver {
         in_lds = true;
         if (g.any(bad_n || badd)) {
             if (r == 0) write_invalid();
+            if constexpr (WIT) zero_row(0);
             return -1;
         }
         Q = uni<G>((1u << n) - 1u);
@@ -1057,6 +1072,14 @@ struct SmallSolver {
         }
         hi = wr = (na ? 1u : 0u) + (nb ? 1u : 0u);
         level = 1;
+        if constexpr (WIT) {
+            if (r == 0 && hi > 0u) {
+                if (na) hist[0] = (uint16_t)root_a;
+                if (nb) hist[na ? 1u : 0u] = (uint16_t)root_b;
+                lvlp[1] = 0u;
+                lvlp[2] = hi;
+            }
+        }
         return hi == 0u ? (int)kNoSync : (int)kContinue;
     }
 
@@ -1084,6 +1107,7 @@ struct SmallSolver {
                         rd = hi;
                         hi = wr;
                         if (rd == hi) return kNoSync;
+                        if constexpr (WIT) record_level();
                     }
                 }
             }
@@ -1333,6 +1357,12 @@ struct SmallSolver {
         bool any_single;
         uint32_t pos[2 * TT], total;
         g.template place<2 * TT>(sm, m, wr, any_single, pos, total);
+        if constexpr (WIT) {
+#pragma unroll
+            for (int j = 0; j < 2 * TT; ++j) {
+                if (nw[j]) hist[pos[j] & 0xFFFFu] = (uint16_t)((j & 1) ? (img[j >> 1] >> 16) : (img[j >> 1] & 0xFFFFu));
+            }
+        }
 #pragma unroll
         for (int j = 0; j < 2 * TT; ++j) {
             const uint32_t val = (j & 1) ? (img[j >> 1] >> 16) : (img[j >> 1] & 0xFFFFu);
@@ -1360,8 +1390,131 @@ struct SmallSolver {
             rd = hi;
             hi = wr;
             if (rd == hi) return kNoSync;
+            if constexpr (WIT) record_level();
         }
         return kContinue;
+    }
+
+    // ---- witness words (WIT) ----
+    __device__ void record_level() {
+        if (g.rank() == 0) lvlp[level + 1u] = hi;
+    }
+    __device__ glb_u8 *row() const { return (glb_u8 *)(P.witness + (uint64_t)ai * (uint64_t)P.witness_stride); }
+    // zero bytes from..stride-1 of the automaton's witness row
+    __device__ void zero_row(int32_t from) const {
+        glb_u8 *out = row();
+        for (int32_t k = from + (int32_t)g.rank(); k < P.witness_stride; k += (int32_t)G) out[k] = 0;
+    }
+    // the LDS bitmap cleared (after the search it holds good marks)
+    __device__ void clear_bitmap() const {
+        const uint32_t words = P.lds_bitmap_words;
+        if (words >= 4) {
+            for (uint32_t w = g.rank(); w < (words >> 2); w += G) *l128(bm_a + 16u * w) = u32x4{0u, 0u, 0u, 0u};
+        } else {
+            for (uint32_t w = g.rank(); w < words; w += G) *l32(bm_a + 4u * w) = 0u;
+        }
+    }
+    __device__ __forceinline__ bool marked(uint32_t y) const {
+        return (*l32(bm_a | ((y >> 3) & 0x1FFCu)) >> (y & 31u)) & 1u;
+    }
+    // The lexicographically smallest shortest reset word (a before b), the
+    // word the reference's back-tracking yields (CPU.cs:262-269).  Backward,
+    // level j = L-1 .. 1: a subset of level j is good when an image of it is a
+    // good subset of level j + 1 (level L - 1: an image is a singleton).  The
+    // LDS bitmap holds the good subsets of level j + 1 only: level j's flags
+    // are all computed (and stored per position) before the bitmap is cleared
+    // and refilled with level j's good subsets, so a mark never leaks across
+    // levels.  Forward: letter a whenever a(cur) is a good subset of the next
+    // level (a search of that level's positions; the bitmap for level 1).
+    __device__ void witness(int32_t L) {
+        constexpr uint32_t U = 4;  // entries per lane per round, loads issued together
+        const uint32_t r = g.rank();
+        wave_sync();  // the recorded subsets are visible to every lane
+        clear_bitmap();
+        lsync();
+        for (int32_t j = L - 1; j >= 1; --j) {
+            const uint32_t s = uni<G>(lvlp[j]), e = uni<G>(lvlp[j + 1]);
+            const bool last = j == L - 1;
+            const bool one_round = e - s <= U * G;
+            uint32_t keep[U] = {};
+            for (uint32_t base = s; base < e; base += U * G) {
+                uint32_t x[U];
+#pragma unroll
+                for (uint32_t u = 0; u < U; ++u) {
+                    const uint32_t p = base + u * G + r;
+                    x[u] = p < e ? (uint32_t)hist[p] : 0u;
+                }
+#pragma unroll
+                for (uint32_t u = 0; u < U; ++u) {
+                    const uint32_t p = base + u * G + r;
+                    const uint32_t im = image(x[u]);
+                    const uint32_t a = im & 0xFFFFu, b = im >> 16;
+                    const bool on = p < e && (last ? (is_single(a) || is_single(b)) : (marked(a) || marked(b)));
+                    if (p < e) goodp[p] = on ? 1u : 0u;
+                    keep[u] = on ? x[u] : 0u;
+                }
+            }
+            lsync();
+            clear_bitmap();
+            lsync();
+            if (one_round) {
+#pragma unroll
+                for (uint32_t u = 0; u < U; ++u)
+                    if (keep[u]) or_wg(l32(bm_a | ((keep[u] >> 3) & 0x1FFCu)), 1u << (keep[u] & 31u));
+            } else {
+                wave_sync();  // this level's flags are visible to every lane
+                for (uint32_t base = s; base < e; base += U * G) {
+                    uint32_t x[U], f[U];
+#pragma unroll
+                    for (uint32_t u = 0; u < U; ++u) {
+                        const uint32_t p = base + u * G + r;
+                        x[u] = p < e ? (uint32_t)hist[p] : 0u;
+                        f[u] = p < e ? (uint32_t)goodp[p] : 0u;
+                    }
+#pragma unroll
+                    for (uint32_t u = 0; u < U; ++u)
+                        if (f[u]) or_wg(l32(bm_a | ((x[u] >> 3) & 0x1FFCu)), 1u << (x[u] & 31u));
+                }
+            }
+            lsync();
+        }
+        wave_sync();  // every level's flags are visible to every lane
+        glb_u8 *out = row();
+        uint32_t cur = Q, acc = 0;
+        for (int32_t k = 0; k < L; ++k) {
+            const uint32_t im = image(cur);
+            const uint32_t a = im & 0xFFFFu, b = im >> 16;
+            bool take_a;
+            if (k == L - 1) {
+                take_a = is_single(a);
+            } else if (k == 0) {
+                take_a = marked(a);  // the bitmap holds level 1's good subsets
+            } else {
+                const uint32_t s = uni<G>(lvlp[k + 1]), e = uni<G>(lvlp[k + 2]);
+                bool hit = false;
+                for (uint32_t base = s; base < e && !hit; base += U * G) {
+                    uint32_t x[U], f[U];
+#pragma unroll
+                    for (uint32_t u = 0; u < U; ++u) {
+                        const uint32_t p = base + u * G + r;
+                        x[u] = p < e ? (uint32_t)hist[p] : 0u;
+                        f[u] = p < e ? (uint32_t)goodp[p] : 0u;
+                    }
+                    bool mine = false;
+#pragma unroll
+                    for (uint32_t u = 0; u < U; ++u) mine |= f[u] != 0u && x[u] == a;
+                    hit = g.any(mine);
+                }
+                take_a = hit;
+            }
+            if (!take_a) acc |= 1u << (k & 7);
+            cur = take_a ? a : b;
+            if ((k & 7) == 7 || k == L - 1) {
+                if (r == 0 && (k >> 3) < P.witness_stride) out[k >> 3] = (uint8_t)acc;
+                acc = 0;
+            }
+        }
+        zero_row((L + 7) >> 3);
     }
 
     __device__ void finish(int code) {
@@ -1372,6 +1525,12 @@ struct SmallSolver {
         const bool found = code == kFound;
         const int32_t L = found ? (int32_t)level + 1 : -1;
         const int64_t E = (int64_t)hi + e0;
+        if constexpr (WIT) {
+            if (found)
+                witness(L);
+            else
+                zero_row(0);
+        }
         if (g.rank() == 0) {
             P.length[ai] = L;
             if (P.sync) P.sync[ai] = found ? 1 : 0;
@@ -1473,7 +1632,9 @@ static hipError_t occ_t(int block, size_t lds, int *blocks) {
 
 // variant: 0 = general (witness, HBM bitmaps); 1..3 = small path with T = 1, 2, 4
 // and nibble tables; 4..6 = small path with T = 1, 2, 4 and byte tables; 7 =
-// small path with T = 8 and byte tables, one wavefront per automaton
+// small path with T = 8 and byte tables, one wavefront per automaton; 8, 9 =
+// small path with witness words (one wavefront per automaton), T = 1 with
+// nibble tables and T = 4 with byte tables
 #define SSW_SMALL_CASES(FN, V, TT, BT, ...)                                                                   \
     case V * 1000 + 16: return FN<SmallSolver<WaveGroup<16>, TT, BT>, WaveGroup<16>>(__VA_ARGS__);            \
     case V * 1000 + 32: return FN<SmallSolver<WaveGroup<32>, TT, BT>, WaveGroup<32>>(__VA_ARGS__);            \
@@ -1493,6 +1654,8 @@ static hipError_t occ_t(int block, size_t lds, int *blocks) {
         SSW_SMALL_CASES(FN, 5, 2, true, __VA_ARGS__)                                       \
         SSW_SMALL_CASES(FN, 6, 4, true, __VA_ARGS__)                                       \
         case 7064: return FN<SmallSolver<WaveGroup<64>, 8, true>, WaveGroup<64>>(__VA_ARGS__); \
+        case 8064: return FN<SmallSolver<WaveGroup<64>, 1, false, true>, WaveGroup<64>>(__VA_ARGS__); \
+        case 9064: return FN<SmallSolver<WaveGroup<64>, 4, true, true>, WaveGroup<64>>(__VA_ARGS__); \
         default: return hipErrorInvalidValue;                                              \
     }
 
@@ -1511,15 +1674,18 @@ size_t batch_scratch_bytes(int group_lanes) {
 
 // LDS bytes of one workgroup of `groups` groups (with the group scratch), and
 // the per-group figure the kernel reads from BatchParams.lds_group_bytes.
+// variants 4..7 and 9 use byte tables, 1..3 and 8 nibble tables
+static bool variant_byte_tab(int variant) { return (variant >= 4 && variant <= 7) || variant == 9; }
+
 size_t batch_group_lds(int variant, uint32_t bitmap_words, uint32_t arena_cap) {
     return variant == 0 ? (size_t)(kTableBytes + 4ull * bitmap_words + 4ull * arena_cap)
-                        : (size_t)SmallLds::region_bytes(variant >= 4, arena_cap);
+                        : (size_t)SmallLds::region_bytes(variant_byte_tab(variant), arena_cap);
 }
 
 size_t batch_block_lds(int variant, int group_lanes, int groups, uint32_t bitmap_words, uint32_t arena_cap) {
     const size_t scratch = batch_scratch_bytes(group_lanes);
     if (variant == 0) return batch_group_lds(0, bitmap_words, arena_cap) * groups + scratch;
-    return SmallLds::block_bytes((uint32_t)groups, 4u * bitmap_words, SmallLds::region_bytes(variant >= 4, arena_cap),
+    return SmallLds::block_bytes((uint32_t)groups, 4u * bitmap_words, SmallLds::region_bytes(variant_byte_tab(variant), arena_cap),
                                  (uint32_t)scratch);
 }
 

@@ -215,8 +215,17 @@ static inline uint32_t next_pow2(uint64_t x) {
 //   2. (narrow_bits only) the same ring with the full 2^n-bit bitmap,
 //   3. the larger ring (spill_arena_lds), byte tables.
 int ssw_context::launch_class(const DeviceBatch &db, const ClassSpec &cls, bool plain) {
-    const ssw_tuning t = plain ? ssw_tuning{256, 256, 1, 8192, 1, 0, 0, 0, 0} : tune[cls.nmax];
-    const bool small = cls.lds && db.witness == nullptr && cls.nmax <= kMaxLdsN;
+    ssw_tuning t = plain ? ssw_tuning{256, 256, 1, 8192, 1, 0, 0, 0, 0} : tune[cls.nmax];
+    const bool small = cls.lds && cls.nmax <= kMaxLdsN && (db.witness == nullptr || !plain);
+    if (small && db.witness != nullptr) {
+        // witness words on the n <= 16 path: one wavefront per automaton,
+        // one entry per lane in the first launches and four in the last
+        t.group_lanes = 64;
+        t.block_threads = 0;
+        t.lane_entries = 1;
+        t.spill_group_lanes = 0;
+        t.spill_lane_entries = 4;
+    }
     if (!plain && !small && db.witness == nullptr && cls.nmax <= SSW_MAX_N && t.narrow_bits == 16 &&
         cls.spill_queue[0]) {
         // n > 16: automata with |a(Q) | b(Q)| <= 16 run relabelled in the
@@ -301,15 +310,20 @@ int ssw_context::launch_pass(const DeviceBatch &db, const ClassSpec &cls, const 
     const bool use_lds = cls.lds || io.force_small;
     const int nmax = cls.nmax;
     const bool want_wit = db.witness != nullptr;
-    // the small path: bitmap in LDS, no witness, 16-bit ring frontier
-    const bool small = io.force_small || (use_lds && !want_wit && nmax <= kMaxLdsN);
+    // the small path: bitmap in LDS, 16-bit ring frontier (with witness
+    // words: one wavefront per automaton, not the plain path)
+    const bool small = io.force_small || (use_lds && nmax <= kMaxLdsN && (!want_wit || (!plain && G == 64)));
     // the last launch of a spill chain takes the wide searches: byte-sliced
     // image tables (two lookups per subset instead of four), and with one
     // wavefront per automaton up to 8 entries per lane
     const bool byte_tab = io.byte_tab;
-    const int T = (t.lane_entries >= 8 && byte_tab && G == 64) ? 8
+    const bool wit_small = small && want_wit;
+    const int T = wit_small ? (byte_tab ? 4 : 1)
+                  : (t.lane_entries >= 8 && byte_tab && G == 64) ? 8
                   : (t.lane_entries >= 4 ? 4 : (t.lane_entries >= 2 ? 2 : 1));
-    const int variant = !small ? 0 : (T == 8 ? 7 : (T == 1 ? 1 : (T == 2 ? 2 : 3)) + (byte_tab ? 3 : 0));
+    const int variant = !small ? 0
+                        : wit_small ? (byte_tab ? 9 : 8)
+                        : (T == 8 ? 7 : (T == 1 ? 1 : (T == 2 ? 2 : 3)) + (byte_tab ? 3 : 0));
 
     BatchParams P;
     std::memset(&P, 0, sizeof P);
@@ -395,7 +409,19 @@ int ssw_context::launch_pass(const DeviceBatch &db, const ClassSpec &cls, const 
         P.garena_off = 0;
         P.garena_cap = 65536;
         off = align_up(2ull * 65536, 256);
-        P.gbitmap_off = P.good_off = P.lvl_off = off;
+        if (wit_small) {
+            // discovery order (u16), a good flag per position (u8) and level
+            // starts (u32) of up to 65536 subsets
+            P.hist_off = off;
+            off += 2ull * 65536;
+            P.good_off = off;
+            off += 65536;
+            P.lvl_off = off;
+            off = align_up(off + 4ull * (65536 + 4), 256);
+            P.gbitmap_off = off;
+        } else {
+            P.hist_off = P.gbitmap_off = P.good_off = P.lvl_off = off;
+        }
     } else {
         // every subset fits the HBM arena when n <= 20; larger automata whose
         // frontier outgrows it are deferred to the device-wide path

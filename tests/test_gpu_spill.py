@@ -154,3 +154,40 @@ def test_relabelled_large_n(ctx):
         np.testing.assert_array_equal(r.length[idx], L, err_msg=f"n={n}")
         np.testing.assert_array_equal(r.sync[idx], S, err_msg=f"n={n}")
         np.testing.assert_array_equal(r.expanded[idx], E, err_msg=f"n={n}")
+
+
+WIT_CONFIGS = [
+    (64, 128, 16, 512, 1, 4096, 4, 0),       # the committed spill chain
+    (64, 0, 4, 256, 1, 2048, 4, 0),          # last launch's ring too small for C_16: HBM ring
+    (64, 0, 2, 2048, 1, 0, 0, 0),            # no spill: frontiers move to the HBM ring
+    (64, 128, 16, 512, 1, 4096, 4, 0, 15),   # relabelled states, three launches
+    (64, 128, 16, 512, 1, 4096, 4, 0, 13),
+]
+
+
+@pytest.fixture(scope="module")
+def words(case):
+    A, B, L, S, E = case
+    return [O.witness(N, A[i], B[i])[1] if L[i] >= 0 else None for i in range(len(A))]
+
+
+@pytest.mark.parametrize("cfg", WIT_CONFIGS, ids=[",".join(map(str, c)) for c in WIT_CONFIGS])
+def test_witness_words_every_configuration(ctx, case, words, cfg):
+    """Witness words from the n <= 16 path (recorded levels, good marks in the
+    LDS bitmap) through every route a search can take: first launch, spill
+    chain, HBM ring."""
+    A, B, L, S, E = case
+    saved = ctx.get_tuning(N)
+    try:
+        ctx.set_tuning(N, *cfg)
+        r = ctx.compute(N, A, B, witness=True)
+    finally:
+        ctx.set_tuning(N, **saved)
+    np.testing.assert_array_equal(r.length, L, err_msg=f"{cfg}: length")
+    np.testing.assert_array_equal(r.expanded, E, err_msg=f"{cfg}: expanded")
+    for i in range(len(A)):
+        if L[i] >= 0:
+            w = ssw.witness_bits_to_word(r.witness[i], int(L[i]))
+            np.testing.assert_array_equal(w, words[i], err_msg=f"{cfg}: automaton {i}")
+        else:
+            assert not r.witness[i].any()
