@@ -1426,14 +1426,22 @@ struct SmallSolver {
     // and refilled with level j's good subsets, so a mark never leaks across
     // levels.  Forward: letter a whenever a(cur) is a good subset of the next
     // level (a search of that level's positions; the bitmap for level 1).
+    // level starts lvlp[1..L] copied into the LDS ring (free after the
+    // search) when they fit, so each level's bounds are an LDS read
+    __device__ uint32_t level_start(uint32_t k, bool cached) const {
+        return cached ? uni<G>(*l32(ring_a + 4u * k)) : uni<G>(lvlp[k]);
+    }
     __device__ void witness(int32_t L) {
         constexpr uint32_t U = 4;  // entries per lane per round, loads issued together
         const uint32_t r = g.rank();
         wave_sync();  // the recorded subsets are visible to every lane
         clear_bitmap();
+        const bool cached = 4u * ((uint32_t)L + 1u) <= 2u * (rcap + 64u);
+        if (cached)
+            for (uint32_t k = r; k <= (uint32_t)L; k += G) *l32(ring_a + 4u * k) = lvlp[k];
         lsync();
         for (int32_t j = L - 1; j >= 1; --j) {
-            const uint32_t s = uni<G>(lvlp[j]), e = uni<G>(lvlp[j + 1]);
+            const uint32_t s = level_start(j, cached), e = level_start(j + 1, cached);
             const bool last = j == L - 1;
             const bool one_round = e - s <= U * G;
             uint32_t keep[U] = {};
@@ -1490,7 +1498,7 @@ struct SmallSolver {
             } else if (k == 0) {
                 take_a = marked(a);  // the bitmap holds level 1's good subsets
             } else {
-                const uint32_t s = uni<G>(lvlp[k + 1]), e = uni<G>(lvlp[k + 2]);
+                const uint32_t s = level_start(k + 1, cached), e = level_start(k + 2, cached);
                 bool hit = false;
                 for (uint32_t base = s; base < e && !hit; base += U * G) {
                     uint32_t x[U], f[U];
