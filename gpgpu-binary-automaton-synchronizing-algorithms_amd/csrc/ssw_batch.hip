@@ -1426,23 +1426,81 @@ struct SmallSolver {
     // and refilled with level j's good subsets, so a mark never leaks across
     // levels.  Forward: letter a whenever a(cur) is a good subset of the next
     // level (a search of that level's positions; the bitmap for level 1).
-    // level starts lvlp[1..L] copied into the LDS ring (free after the
-    // search) when they fit, so each level's bounds are an LDS read
-    __device__ uint32_t level_start(uint32_t k, bool cached) const {
-        return cached ? uni<G>(*l32(ring_a + 4u * k)) : uni<G>(lvlp[k]);
+    // Where the passes find the recorded search.  When the whole history is
+    // still in the LDS ring (no wrap, never moved to HBM) and there is room
+    // after it, the flags and level starts go there too and the passes touch
+    // no HBM; otherwise the history and flags are read from the HBM slot and
+    // only the level starts are copied into the ring area.
+    struct WitnessSpace {
+        bool local;
+        uint32_t flags_a, starts_a;  // LDS byte addresses (local: flags and starts; else starts only)
+        bool starts_cached;
+    };
+    __device__ WitnessSpace witness_space(int32_t L) const {
+        WitnessSpace w;
+        const uint32_t room = 2u * (rcap + 64u);
+        const uint32_t fo = (2u * wr + 3u) & ~3u;
+        const uint32_t so = (fo + wr + 3u) & ~3u;
+        w.local = in_lds && wr <= rcap && so + 4u * ((uint32_t)L + 1u) <= room;
+        w.flags_a = ring_a + fo;
+        w.starts_a = w.local ? ring_a + so : ring_a;
+        w.starts_cached = w.local || 4u * ((uint32_t)L + 1u) <= room;
+        return w;
+    }
+    __device__ __forceinline__ uint32_t level_start(const WitnessSpace &w, uint32_t k) const {
+        return w.starts_cached ? uni<G>(*l32(w.starts_a + 4u * k)) : uni<G>(lvlp[k]);
+    }
+    __device__ __forceinline__ uint32_t hist_at(const WitnessSpace &w, uint32_t p) const {
+        return w.local ? (uint32_t)*l16(ring_a + 2u * p) : (uint32_t)hist[p];
+    }
+    __device__ __forceinline__ uint32_t flag_at(const WitnessSpace &w, uint32_t p) const {
+        return w.local ? (uint32_t)*l8(w.flags_a + p) : (uint32_t)goodp[p];
+    }
+    __device__ __forceinline__ void set_flag(const WitnessSpace &w, uint32_t p, bool on) const {
+        if (w.local)
+            *l8(w.flags_a + p) = on ? 1u : 0u;
+        else
+            goodp[p] = on ? 1u : 0u;
     }
     __device__ void witness(int32_t L) {
         constexpr uint32_t U = 4;  // entries per lane per round, loads issued together
         const uint32_t r = g.rank();
+        const WitnessSpace w = witness_space(L);
         wave_sync();  // the recorded subsets are visible to every lane
         clear_bitmap();
-        const bool cached = 4u * ((uint32_t)L + 1u) <= 2u * (rcap + 64u);
-        if (cached)
-            for (uint32_t k = r; k <= (uint32_t)L; k += G) *l32(ring_a + 4u * k) = lvlp[k];
+        if (w.starts_cached)
+            for (uint32_t k = r; k <= (uint32_t)L; k += G) *l32(w.starts_a + 4u * k) = lvlp[k];
         lsync();
+        // a level of at most G entries (most of them) takes one entry per
+        // lane, and the bitmap is then cleared bit by bit from the marks it
+        // set (prev) instead of wholesale
+        bool prev_narrow = false;
+        uint32_t prev = 0;
         for (int32_t j = L - 1; j >= 1; --j) {
-            const uint32_t s = level_start(j, cached), e = level_start(j + 1, cached);
+            const uint32_t s = level_start(w, j), e = level_start(w, j + 1);
             const bool last = j == L - 1;
+            if (e - s <= G) {
+                const uint32_t p = s + r;
+                const uint32_t x = p < e ? hist_at(w, p) : 0u;
+                const uint32_t im = image(x);
+                const uint32_t a = im & 0xFFFFu, b = im >> 16;
+                const bool on = p < e && (last ? (is_single(a) || is_single(b)) : (marked(a) || marked(b)));
+                if (p < e) set_flag(w, p, on);
+                lsync();
+                if (prev_narrow) {
+                    if (prev) __hip_atomic_fetch_and(l32(bm_a | ((prev >> 3) & 0x1FFCu)), ~(1u << (prev & 31u)),
+                                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                } else {
+                    clear_bitmap();
+                }
+                lsync();
+                if (on) or_wg(l32(bm_a | ((x >> 3) & 0x1FFCu)), 1u << (x & 31u));
+                prev = on ? x : 0u;
+                prev_narrow = true;
+                lsync();
+                continue;
+            }
+            prev_narrow = false;
             const bool one_round = e - s <= U * G;
             uint32_t keep[U] = {};
             for (uint32_t base = s; base < e; base += U * G) {
@@ -1450,7 +1508,7 @@ struct SmallSolver {
 #pragma unroll
                 for (uint32_t u = 0; u < U; ++u) {
                     const uint32_t p = base + u * G + r;
-                    x[u] = p < e ? (uint32_t)hist[p] : 0u;
+                    x[u] = p < e ? hist_at(w, p) : 0u;
                 }
 #pragma unroll
                 for (uint32_t u = 0; u < U; ++u) {
@@ -1458,7 +1516,7 @@ struct SmallSolver {
                     const uint32_t im = image(x[u]);
                     const uint32_t a = im & 0xFFFFu, b = im >> 16;
                     const bool on = p < e && (last ? (is_single(a) || is_single(b)) : (marked(a) || marked(b)));
-                    if (p < e) goodp[p] = on ? 1u : 0u;
+                    if (p < e) set_flag(w, p, on);
                     keep[u] = on ? x[u] : 0u;
                 }
             }
@@ -1470,14 +1528,14 @@ struct SmallSolver {
                 for (uint32_t u = 0; u < U; ++u)
                     if (keep[u]) or_wg(l32(bm_a | ((keep[u] >> 3) & 0x1FFCu)), 1u << (keep[u] & 31u));
             } else {
-                wave_sync();  // this level's flags are visible to every lane
+                if (!w.local) wave_sync();  // this level's flags are visible to every lane
                 for (uint32_t base = s; base < e; base += U * G) {
                     uint32_t x[U], f[U];
 #pragma unroll
                     for (uint32_t u = 0; u < U; ++u) {
                         const uint32_t p = base + u * G + r;
-                        x[u] = p < e ? (uint32_t)hist[p] : 0u;
-                        f[u] = p < e ? (uint32_t)goodp[p] : 0u;
+                        x[u] = p < e ? hist_at(w, p) : 0u;
+                        f[u] = p < e ? flag_at(w, p) : 0u;
                     }
 #pragma unroll
                     for (uint32_t u = 0; u < U; ++u)
@@ -1486,7 +1544,7 @@ struct SmallSolver {
             }
             lsync();
         }
-        wave_sync();  // every level's flags are visible to every lane
+        if (!w.local) wave_sync();  // every level's flags are visible to every lane
         glb_u8 *out = row();
         uint32_t cur = Q, acc = 0;
         for (int32_t k = 0; k < L; ++k) {
@@ -1498,15 +1556,19 @@ struct SmallSolver {
             } else if (k == 0) {
                 take_a = marked(a);  // the bitmap holds level 1's good subsets
             } else {
-                const uint32_t s = level_start(k + 1, cached), e = level_start(k + 2, cached);
+                const uint32_t s = level_start(w, k + 1), e = level_start(w, k + 2);
                 bool hit = false;
-                for (uint32_t base = s; base < e && !hit; base += U * G) {
+                if (e - s <= G) {
+                    const uint32_t p = s + r;
+                    hit = g.any(p < e && flag_at(w, p) != 0u && hist_at(w, p) == a);
+                }
+                for (uint32_t base = s; base < e && e - s > G && !hit; base += U * G) {
                     uint32_t x[U], f[U];
 #pragma unroll
                     for (uint32_t u = 0; u < U; ++u) {
                         const uint32_t p = base + u * G + r;
-                        x[u] = p < e ? (uint32_t)hist[p] : 0u;
-                        f[u] = p < e ? (uint32_t)goodp[p] : 0u;
+                        x[u] = p < e ? hist_at(w, p) : 0u;
+                        f[u] = p < e ? flag_at(w, p) : 0u;
                     }
                     bool mine = false;
 #pragma unroll
