@@ -191,3 +191,32 @@ def test_witness_words_every_configuration(ctx, case, words, cfg):
             np.testing.assert_array_equal(w, words[i], err_msg=f"{cfg}: automaton {i}")
         else:
             assert not r.witness[i].any()
+
+
+def test_witness_words_mixed_sizes(ctx):
+    """One call over n = 10..20 with witness words: the n <= 16 classes run
+    the small kernels' witness passes, the others the general kernel, on
+    their own streams at the same time."""
+    rng = np.random.default_rng(29)
+    ns = rng.integers(10, 21, size=3000).astype(np.uint8)
+    As, Bs = [], []
+    for i, n in enumerate(ns):
+        if i % 101 == 0:
+            a, b = ssw.generate_cerny(int(n), 1)
+        else:
+            a = rng.integers(0, n, size=n)
+            b = rng.integers(0, n, size=n)
+        As.append(np.asarray(a, np.int32))
+        Bs.append(np.asarray(b, np.int32))
+    r = ctx.solve_mixed(ns, As, Bs, witness=True)
+    for n in np.unique(ns):
+        idx = np.nonzero(ns == n)[0]
+        A = np.stack([As[i] for i in idx])
+        B = np.stack([Bs[i] for i in idx])
+        L, S, E, _ = O.solve(int(n), A, B, threads=8)
+        np.testing.assert_array_equal(r.length[idx], L, err_msg=f"n={n}")
+        np.testing.assert_array_equal(r.expanded[idx], E, err_msg=f"n={n}")
+        for k, i in enumerate(idx):
+            if L[k] >= 0:
+                w = ssw.witness_bits_to_word(r.witness[i], int(L[k]))
+                np.testing.assert_array_equal(w, O.witness(int(n), A[k], B[k])[1], err_msg=f"n={n} i={i}")
