@@ -1417,15 +1417,6 @@ struct SmallSolver {
     __device__ __forceinline__ bool marked(uint32_t y) const {
         return (*l32(bm_a | ((y >> 3) & 0x1FFCu)) >> (y & 31u)) & 1u;
     }
-    // The lexicographically smallest shortest reset word (a before b), the
-    // word the reference's back-tracking yields (CPU.cs:262-269).  Backward,
-    // level j = L-1 .. 1: a subset of level j is good when an image of it is a
-    // good subset of level j + 1 (level L - 1: an image is a singleton).  The
-    // LDS bitmap holds the good subsets of level j + 1 only: level j's flags
-    // are all computed (and stored per position) before the bitmap is cleared
-    // and refilled with level j's good subsets, so a mark never leaks across
-    // levels.  Forward: letter a whenever a(cur) is a good subset of the next
-    // level (a search of that level's positions; the bitmap for level 1).
     // Where the passes find the recorded search.  When the whole history is
     // still in the LDS ring (no wrap, never moved to HBM) and there is room
     // after it, the flags and level starts go there too and the passes touch
@@ -1462,6 +1453,15 @@ struct SmallSolver {
         else
             goodp[p] = on ? 1u : 0u;
     }
+    // The lexicographically smallest shortest reset word (a before b), the
+    // word the reference's back-tracking yields (CPU.cs:262-269).  Backward,
+    // level j = L-1 .. 1: a subset of level j is good when an image of it is a
+    // good subset of level j + 1 (level L - 1: an image is a singleton).  The
+    // LDS bitmap holds the good subsets of level j + 1 only: level j's flags
+    // are all computed (and stored per position) before the bitmap is cleared
+    // and refilled with level j's good subsets, so a mark never leaks across
+    // levels.  Forward: letter a whenever a(cur) is a good subset of the next
+    // level (a search of that level's positions; the bitmap for level 1).
     __device__ void witness(int32_t L) {
         constexpr uint32_t U = 4;  // entries per lane per round, loads issued together
         const uint32_t r = g.rank();
