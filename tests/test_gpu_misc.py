@@ -145,6 +145,21 @@ def test_small_budget_rejects_what_cannot_fit():
         # small automata still solve under the same budget
         A, B = ssw.generate_uniform(5, 12, 2000)
         np.testing.assert_array_equal(c.compute(12, A, B).length, O.slim(12, A, B)[0])
+        # witness words need the largest slots (recorded levels per group):
+        # the grid shrinks to the budget instead of failing
+        for n, count in ((12, 300), (16, 40)):
+            A, B = ssw.generate_uniform(6, n, count)
+            ca, cb = ssw.generate_cerny(n, 1)
+            A = np.concatenate([A, ca[None]]).astype(np.int32)
+            B = np.concatenate([B, cb[None]]).astype(np.int32)
+            r = c.compute(n, A, B, witness=True)
+            L = O.slim(n, A, B)[0]
+            np.testing.assert_array_equal(r.length, L, err_msg=f"n={n}")
+            for i in range(len(A)):
+                if L[i] >= 0:
+                    w = ssw.witness_bits_to_word(r.witness[i], int(L[i]))
+                    np.testing.assert_array_equal(w, O.witness(n, A[i], B[i])[1],
+                                                  err_msg=f"n={n} automaton {i}")
 
 
 def test_verify_bound_zero():
