@@ -849,19 +849,26 @@ int ssw_context::solve_entry(const ssw_batch *b, int32_t memory, int mode, int64
 }
 
 // Packs automata [lo, hi) of the int32 arrays into the staging bytes,
-// checking every transition; returns the first bad automaton or -1.
+// checking every transition; returns the first bad automaton or -1.  The
+// check is a running unsigned maximum, so the copy loop vectorises; only a
+// range holding a bad transition is scanned again to find the first one.
 static int64_t pack_range(int32_t n, const int32_t *A, const int32_t *B, uint8_t *packed, int64_t lo, int64_t hi) {
+    const size_t row = (size_t)n;
+    uint32_t most = 0;
     for (int64_t i = lo; i < hi; ++i) {
-        uint8_t *d = packed + (size_t)i * 2 * n;
-        bool ok = true;
-        for (int q = 0; q < n; ++q) {
-            const int32_t x = A[i * n + q], y = B[i * n + q];
-            ok &= (uint32_t)x < (uint32_t)n && (uint32_t)y < (uint32_t)n;
+        uint8_t *d = packed + (size_t)i * 2 * row;
+        const int32_t *a = A + (size_t)i * row, *b = B + (size_t)i * row;
+        for (size_t q = 0; q < row; ++q) {
+            const uint32_t x = (uint32_t)a[q], y = (uint32_t)b[q];
+            most = std::max(most, std::max(x, y));
             d[q] = (uint8_t)x;
-            d[n + q] = (uint8_t)y;
+            d[row + q] = (uint8_t)y;
         }
-        if (!ok) return i;
     }
+    if (most < (uint32_t)n) return -1;
+    for (int64_t i = lo; i < hi; ++i)
+        for (size_t q = 0; q < row; ++q)
+            if ((uint32_t)A[i * row + q] >= (uint32_t)n || (uint32_t)B[i * row + q] >= (uint32_t)n) return i;
     return -1;
 }
 
@@ -879,8 +886,11 @@ int ssw_context::compute_i32(int32_t n, int64_t count, const int32_t *A, const i
     int rc = ensure_stage((size_t)count * 2 * n);
     if (rc) return rc;
     uint8_t *packed = (uint8_t *)h_stage;
-    constexpr int64_t kMinChunk = 1 << 17;
-    const int64_t nchunks = std::max<int64_t>(1, std::min<int64_t>(8, count / kMinChunk));
+    // few large chunks: every solve pays its launches' tails (the longest
+    // searches of the chunk), so a chunk below ~2^18 automata costs more in
+    // tails than the packing it hides
+    constexpr int64_t kMinChunk = 1 << 18;
+    const int64_t nchunks = std::max<int64_t>(1, std::min<int64_t>(4, count / kMinChunk));
     auto chunk_lo = [&](int64_t k) { return count * k / nchunks; };
     const int64_t hw = std::max<int64_t>(1, (int64_t)std::thread::hardware_concurrency());
     const int workers = (int)std::max<int64_t>(1, std::min<int64_t>(hw, 16) - (nchunks > 1 ? 1 : 0));

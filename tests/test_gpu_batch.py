@@ -138,7 +138,7 @@ def test_host_arrays_chunked(ctx):
     output equals the oracle, witnesses included, and the violating index is
     the smallest over all chunks."""
     n = 12
-    count = 300_000
+    count = 600_000  # two chunks of 300,000
     A, B = ssw.generate_uniform(4242, n, count)
     L, S, E, v = O.slim(n, A, B, bound=9, threads=8)
     r = ctx.compute(n, A, B, witness=True, bound=9)
@@ -163,9 +163,15 @@ def test_host_arrays_chunked(ctx):
 
 def test_host_arrays_bad_transition_in_last_chunk(ctx):
     n = 10
-    A, B = ssw.generate_uniform(77, n, 300_000)
-    B[299_990, 4] = n
-    with pytest.raises(ssw.SswError, match="automaton 299990 state 4"):
+    A, B = ssw.generate_uniform(77, n, 600_000)  # two chunks
+    B[599_990, 4] = n
+    with pytest.raises(ssw.SswError, match="automaton 599990 state 4"):
+        ctx.compute(n, A, B)
+    B[599_990, 4] = 0
+    # a negative transition in the first chunk, after a valid run of the same range
+    A[123_456, 7] = -1
+    B[123_457, 0] = n + 3
+    with pytest.raises(ssw.SswError, match="automaton 123456 state 7"):
         ctx.compute(n, A, B)
     # the context stays usable
     r = ctx.compute(n, A[:1000], B[:1000])
@@ -325,7 +331,7 @@ def test_host_arrays_chunked_n20(ctx):
     # the chunked host path for n > 16 (HBM bitmaps, hand-off of long
     # frontiers to the device-wide path) against the oracle
     n = 20
-    count = 270_000
+    count = 540_000  # two chunks
     A, B = ssw.generate_uniform(2020, n, count)
     L, S, E, _ = O.solve(n, A, B, threads=8)
     r = ctx.compute(n, A, B)
