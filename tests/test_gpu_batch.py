@@ -199,7 +199,7 @@ def test_device_batch_marks_invalid_automata(ctx, mode):
     A[3, 2] = n
     B[40, 8] = 200
     delta = ssw.pack_uniform(A, B).reshape(-1)
-    with pytest.raises(ssw.SswError):
+    with pytest.raises(ssw.SswError, match="automaton 3:"):
         ctx.solve_packed(delta, 64, n, mode=mode)
     count = 64
     d = ctx.to_device(delta)
@@ -325,6 +325,29 @@ def test_wide_witness_rows_zero_padded(ctx, mode):
         if L[i] > 0:
             w = ssw.witness_bits_to_word(wit[i], int(L[i]))
             np.testing.assert_array_equal(w, O.witness(n, A[i], B[i])[1])
+
+
+def test_packed_host_rows_first_bad_reported(ctx):
+    """Packed host rows are checked on several host threads before anything is
+    copied: the first bad automaton over all ranges is reported (not the first
+    range's), and the next calls on the same context are solved exactly."""
+    n = 12
+    count = 200_000
+    A, B = ssw.generate_uniform(31, n, count)
+    delta = ssw.pack_uniform(A, B).reshape(-1)
+    bad = delta.copy()
+    bad[190_000 * 2 * n + 5] = 255
+    bad[150_001 * 2 * n + n + 11] = n
+    with pytest.raises(ssw.SswError, match="automaton 150001:"):
+        ctx.solve_packed(bad, count, n)
+    r = ctx.solve_packed(delta, count, n)
+    L, S, E, _ = O.slim(n, A, B, threads=8)
+    np.testing.assert_array_equal(r.length, L)
+    np.testing.assert_array_equal(r.sync, S)
+    np.testing.assert_array_equal(r.expanded, E)
+    # a smaller batch after a larger one
+    r = ctx.solve_packed(delta[:2 * n * 1000], 1000, n, witness=True)
+    np.testing.assert_array_equal(r.length, L[:1000])
 
 
 def test_host_arrays_chunked_n20(ctx):
