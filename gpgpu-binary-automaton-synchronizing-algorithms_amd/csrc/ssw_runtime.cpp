@@ -889,8 +889,10 @@ int ssw_context::compute_i32(int32_t n, int64_t count, const int32_t *A, const i
     // few large chunks: every solve pays its launches' tails (the longest
     // searches of the chunk), so a chunk below ~2^18 automata costs more in
     // tails than the packing it hides
+    // Witness words make the tails far longer (the longest searches also run
+    // the witness passes), so those calls are solved whole.
     constexpr int64_t kMinChunk = 1 << 18;
-    const int64_t nchunks = std::max<int64_t>(1, std::min<int64_t>(4, count / kMinChunk));
+    const int64_t nchunks = witness_out ? 1 : std::max<int64_t>(1, std::min<int64_t>(4, count / kMinChunk));
     auto chunk_lo = [&](int64_t k) { return count * k / nchunks; };
     const int64_t hw = std::max<int64_t>(1, (int64_t)std::thread::hardware_concurrency());
     const int workers = (int)std::max<int64_t>(1, std::min<int64_t>(hw, 16) - (nchunks > 1 ? 1 : 0));
