@@ -443,11 +443,14 @@ __global__ void init_queues_kernel(uint32_t *q0, uint32_t *q1, uint32_t *q2, int
     }
 }
 
-// The per-solve counters: deferred count <- 0, smallest violating index <- none.
-__global__ void reset_counters_kernel(uint32_t *deferred, unsigned long long *violation) {
+// The per-solve counters: deferred count <- 0, smallest violating index <-
+// none, automata done in this solve <- 0.
+__global__ void reset_counters_kernel(uint32_t *deferred, unsigned long long *violation,
+                                      unsigned long long *call_done) {
     if (threadIdx.x == 0) {
         *deferred = 0u;
         *violation = ~0ull;
+        *call_done = 0ull;
     }
 }
 
@@ -773,6 +776,7 @@ struct GroupSolver {
         if (g.rank() == 0 && done_local) {
             atomicAdd(&P.progress[0], (unsigned long long)done_local);
             atomicAdd(&P.progress[1], (unsigned long long)subsets_local);
+            atomicAdd(&P.progress[2], (unsigned long long)done_local);
         }
         done_local = 0;
         subsets_local = 0;
@@ -1618,6 +1622,7 @@ struct SmallSolver {
         if (g.rank() == 0 && done_local) {
             atomicAdd(&P.progress[0], (unsigned long long)done_local);
             atomicAdd(&P.progress[1], (unsigned long long)subsets_local);
+            atomicAdd(&P.progress[2], (unsigned long long)done_local);
         }
         done_local = 0;
         subsets_local = 0;
@@ -1769,8 +1774,9 @@ hipError_t init_queues(uint32_t *queues, int64_t count, hipStream_t st, bool emp
     return init_queues3(queues, nullptr, nullptr, count, empty ? 1u : 0u, st);
 }
 
-hipError_t reset_counters(uint32_t *deferred, unsigned long long *violation, hipStream_t st) {
-    hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, st, deferred, violation);
+hipError_t reset_counters(uint32_t *deferred, unsigned long long *violation, unsigned long long *call_done,
+                          hipStream_t st) {
+    hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, st, deferred, violation, call_done);
     return hipGetLastError();
 }
 

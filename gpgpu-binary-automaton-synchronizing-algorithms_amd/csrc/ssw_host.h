@@ -5,6 +5,7 @@
 #include "ssw_internal.h"
 
 #include <atomic>
+#include <condition_variable>
 #include <cstdarg>
 #include <map>
 #include <mutex>
@@ -23,7 +24,8 @@ enum SolveMode { kModeBatch = 0, kModePlain = 1, kModeLarge = 2 };
 constexpr size_t kCtrWork = 0;         // u32 work-queue head
 constexpr size_t kCtrDeferred = 4;     // u32 deferred count
 constexpr size_t kCtrViolation = 8;    // u64 smallest violating index
-constexpr size_t kCtrProgress = 16;    // u64[2] automata done, subsets expanded
+constexpr size_t kCtrProgress = 16;    // u64[3] automata done, subsets expanded (all solves),
+                                       // automata done in the running solve
 constexpr size_t kCtrLarge = 64;       // device-wide engine scratch counters
 constexpr size_t kCtrBytes = 256;
 
@@ -129,7 +131,8 @@ struct LargeEngine {
     int setup(ssw_context &ctx);
     int set_outputs(ssw_context &ctx, int32_t *length, uint8_t *sync, int64_t *expanded,
                     unsigned long long *violation, unsigned long long *progress);
-    static uint64_t admit_bytes(int n, bool append, uint64_t ring_cap);
+    static uint64_t first_ring_cap(int n, bool append);
+    static uint64_t admit_bytes(int n, bool append);
     // Admits one automaton; *slot_out = -1 when no slot or memory is free now.
     int admit(ssw_context &ctx, int64_t index, int n, const uint8_t *delta, const uint64_t *offset,
               int32_t bound, bool report, bool append, uint64_t budget, int *slot_out);
@@ -183,11 +186,22 @@ struct ssw_context {
     std::atomic<uint64_t> total_launches{0};
     std::atomic<uint64_t> large_done{0};
     std::atomic<uint64_t> large_subsets{0};
-    std::atomic<bool> solving{false};
+    // solve generation: bumped when a solve starts and again when it ends
+    // (odd while one runs), so the progress thread sees every solve, however
+    // short; call_count = automata of the running solve
+    std::atomic<uint64_t> solve_gen{0};
+    std::atomic<uint64_t> call_count{0};
+    void solve_begin(uint64_t count) {
+        call_count.store(count);
+        solve_gen.fetch_add(1);
+    }
+    void solve_end() { solve_gen.fetch_add(1); }
 
     // progress callback
     std::thread prog_thread;
-    std::atomic<bool> prog_stop{false};
+    std::mutex prog_mu;
+    std::condition_variable prog_cv;
+    bool prog_stop = false;  // guarded by prog_mu
     ssw_progress_fn prog_fn = nullptr;
     void *prog_user = nullptr;
     int prog_ms = 100;
@@ -213,6 +227,7 @@ struct ssw_context {
     bool host_checked = false;
     bool host_no_length = false;
     int solve_entry(const ssw_batch *b, int32_t memory, int mode, int64_t *violation_out);
+    int solve_run(const ssw_batch *b, int32_t memory, int mode, int64_t *violation_out);
     int compute_i32(int32_t n, int64_t count, const int32_t *A, const int32_t *B, int32_t *length_out,
                     uint8_t *sync_out, uint8_t *witness_out, int32_t witness_stride, int64_t *expanded_out,
                     int32_t bound, int64_t *violation_out);

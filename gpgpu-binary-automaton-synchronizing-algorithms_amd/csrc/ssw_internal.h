@@ -70,7 +70,7 @@ struct BatchParams {
     uint32_t *queues;                  // kQueues segment counters + exhausted mask
     unsigned int *deferred_count;
     uint32_t *deferred_list;           // automata whose frontier outgrew the arena
-    unsigned long long *progress;      // [0] automata done, [1] subsets expanded
+    unsigned long long *progress;      // [0] automata done, [1] subsets expanded, [2] done in this solve
     uint8_t *slot_base;                // per-group global scratch
     uint64_t slot_bytes;
     uint64_t garena_off;               // u32 frontier arena beyond the LDS part

@@ -96,6 +96,7 @@ __device__ void lg_report(LargeState *st, LargeInstance &I) {
     if (st->progress && (I.status == kLFound || I.status == kLNoSync)) {
         atomicAdd(&st->progress[0], 1ull);
         atomicAdd(&st->progress[1], (unsigned long long)I.expanded);
+        atomicAdd(&st->progress[2], 1ull);
     }
 }
 

@@ -33,7 +33,8 @@ hipError_t fill_u32(uint32_t *p, uint64_t words, uint32_t value, hipStream_t st)
 hipError_t init_queues(uint32_t *queues, int64_t count, hipStream_t st, bool empty);
 hipError_t init_queues3(uint32_t *q0, uint32_t *q1, uint32_t *q2, int64_t count, uint32_t empty_mask,
                         hipStream_t st);
-hipError_t reset_counters(uint32_t *deferred, unsigned long long *violation, hipStream_t st);
+hipError_t reset_counters(uint32_t *deferred, unsigned long long *violation, unsigned long long *call_done,
+                          hipStream_t st);
 }  // namespace ssw
 
 using namespace ssw;
@@ -495,9 +496,10 @@ int ssw_context::solve_device(const DeviceBatch &db, const uint8_t *host_n, bool
     ++call_gen;
     int rc = ensure(deferred, 4ull * std::max<int64_t>(1, db.count), "deferred list");
     if (rc) return rc;
-    // violation <- UINT64_MAX, deferred count <- 0
+    // violation <- UINT64_MAX, deferred count <- 0, done in this solve <- 0
     HIP_TRY(reset_counters((uint32_t *)((uint8_t *)ctrs.p + kCtrDeferred),
-                           (unsigned long long *)((uint8_t *)ctrs.p + kCtrViolation), stream),
+                           (unsigned long long *)((uint8_t *)ctrs.p + kCtrViolation),
+                           (unsigned long long *)((uint8_t *)ctrs.p + kCtrProgress) + 2, stream),
             "reset counters");
     HIP_TRY(hipEventRecord(ev0, stream), "hipEventRecord");
 
@@ -807,7 +809,13 @@ int ssw_context::solve_entry(const ssw_batch *b, int32_t memory, int mode, int64
     if (violation_out) *violation_out = -1;
     if (b->count == 0) return SSW_OK;
     HIP_TRY(hipSetDevice(device), "hipSetDevice");
-    solving.store(true);
+    solve_begin((uint64_t)b->count);
+    const int rc = solve_run(b, memory, mode, violation_out);
+    solve_end();
+    return rc;
+}
+
+int ssw_context::solve_run(const ssw_batch *b, int32_t memory, int mode, int64_t *violation_out) {
     int rc;
     if (memory == SSW_HOST) {
         rc = solve_host_packed(b, mode, violation_out);
@@ -828,15 +836,10 @@ int ssw_context::solve_entry(const ssw_batch *b, int32_t memory, int mode, int64
         if (b->uniform_n <= 0) {
             host_n.resize(b->count);
             hipError_t e = hipMemcpy(host_n.data(), b->n_per, b->count, hipMemcpyDeviceToHost);
-            if (e != hipSuccess) {
-                solving.store(false);
-                return hip_fail(e, "D2H n_per");
-            }
+            if (e != hipSuccess) return hip_fail(e, "D2H n_per");
             for (int64_t i = 0; i < b->count; ++i)
-                if (host_n[i] < 1 || host_n[i] > SSW_MAX_N) {
-                    solving.store(false);
+                if (host_n[i] < 1 || host_n[i] > SSW_MAX_N)
                     return fail(SSW_ERR_ARG, "automaton %lld: n=%d out of [1,30]", (long long)i, (int)host_n[i]);
-                }
         }
         const uint8_t *hn = b->uniform_n > 0 ? nullptr : host_n.data();
         rc = mode == kModeLarge ? large.solve_all(*this, db, hn, violation_out)
@@ -844,7 +847,6 @@ int ssw_context::solve_entry(const ssw_batch *b, int32_t memory, int mode, int64
     } else {
         rc = fail(SSW_ERR_ARG, "memory kind %d", memory);
     }
-    solving.store(false);
     return rc;
 }
 
@@ -981,41 +983,62 @@ int ssw_context::compute_i32(int32_t n, int64_t count, const int32_t *A, const i
 
 int ssw_context::read_progress(ssw_progress *out) {
     std::lock_guard<std::mutex> lk(monitor_mu);
-    unsigned long long v[2] = {0, 0};
+    // a solve running from the first read to the second: its count is the
+    // one the done-in-this-solve counter belongs to
+    const uint64_t g0 = solve_gen.load();
+    const uint64_t cnt = call_count.load();
+    unsigned long long v[3] = {0, 0, 0};
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess)
         e = hipMemcpyAsync(v, (uint8_t *)ctrs.p + kCtrProgress, sizeof v, hipMemcpyDeviceToHost, monitor);
     if (e == hipSuccess) e = hipStreamSynchronize(monitor);
     if (e != hipSuccess) return SSW_ERR_DEVICE;
+    const bool running = (g0 & 1u) && solve_gen.load() == g0;
     out->automata_done = v[0] + large_done.load();
     out->subsets_expanded = v[1] + large_subsets.load();
     out->launches = total_launches.load();
     out->deferred = deferred_total;
+    out->in_flight = running && cnt > v[2] ? cnt - v[2] : 0;
     return SSW_OK;
 }
 
+// The thread looks every prog_ms: it reports while a solve runs (odd
+// generation) and whenever the generation moved since its last look, so a
+// solve that started and ended between two looks is still reported, with its
+// final counts.  Stopping makes one last look.
 void ssw_context::start_progress_thread() {
     stop_progress_thread();
-    prog_stop.store(false);
+    {
+        std::lock_guard<std::mutex> lk(prog_mu);
+        prog_stop = false;
+    }
     prog_thread = std::thread([this] {
-        bool was_solving = false;
-        while (!prog_stop.load()) {
-            std::this_thread::sleep_for(std::chrono::milliseconds(prog_ms));
-            const bool now = solving.load();
-            if (now || was_solving) {
+        uint64_t seen = solve_gen.load();
+        std::unique_lock<std::mutex> lk(prog_mu);
+        for (;;) {
+            const bool stopping =
+                prog_cv.wait_for(lk, std::chrono::milliseconds(prog_ms), [this] { return prog_stop; });
+            const uint64_t g = solve_gen.load();
+            if ((g & 1u) || g != seen) {
+                lk.unlock();
                 ssw_progress p;
-                if (read_progress(&p) == SSW_OK && prog_fn) prog_fn(&p, prog_user);
+                if (read_progress(&p) == SSW_OK) prog_fn(&p, prog_user);
+                lk.lock();
             }
-            was_solving = now;
+            seen = g;
+            if (stopping) return;
         }
     });
 }
 
 void ssw_context::stop_progress_thread() {
-    if (prog_thread.joinable()) {
-        prog_stop.store(true);
-        prog_thread.join();
+    if (!prog_thread.joinable()) return;
+    {
+        std::lock_guard<std::mutex> lk(prog_mu);
+        prog_stop = true;
     }
+    prog_cv.notify_all();
+    prog_thread.join();
 }
 
 // ---------------------------------------------------------------------------

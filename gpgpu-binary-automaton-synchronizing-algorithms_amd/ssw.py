@@ -75,13 +75,13 @@ class Tuning(ctypes.Structure):
 
 class Progress(ctypes.Structure):
     _fields_ = [("automata_done", ctypes.c_uint64), ("subsets_expanded", ctypes.c_uint64),
-                ("launches", ctypes.c_uint64), ("deferred", ctypes.c_uint64)]
+                ("launches", ctypes.c_uint64), ("deferred", ctypes.c_uint64), ("in_flight", ctypes.c_uint64)]
 
 
 class ServerConfig(ctypes.Structure):
     _fields_ = [("mem_budget", ctypes.c_uint64), ("small_n_max", ctypes.c_int32),
                 ("levels_per_tick", ctypes.c_int32), ("max_large", ctypes.c_int32),
-                ("want_witness", ctypes.c_int32)]
+                ("want_witness", ctypes.c_int32), ("small_workers", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 class Result(ctypes.Structure):
@@ -156,6 +156,11 @@ def lib():
 
 def _ptr(a):
     return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _progress_dict(p):
+    return dict(automata_done=p.automata_done, subsets_expanded=p.subsets_expanded, launches=p.launches,
+                deferred=p.deferred, in_flight=p.in_flight)
 
 
 # ---------------------------------------------------------------- generators
@@ -399,31 +404,35 @@ class Context:
     def progress(self):
         p = Progress()
         self._check(self._L.ssw_get_progress(self.h, ctypes.byref(p)), "progress")
-        return dict(automata_done=p.automata_done, subsets_expanded=p.subsets_expanded,
-                    launches=p.launches, deferred=p.deferred)
+        return _progress_dict(p)
 
     def set_progress_callback(self, fn, interval_ms=50):
+        """Calls fn(progress dict) from the library's thread (None: stop).
+
+        The library joins its previous thread before it returns, so the old
+        trampoline is released only after that call: the thread may be inside
+        it until then."""
         if fn is None:
-            self._cb = None
             self._check(self._L.ssw_set_progress_callback(self.h, PROGRESS_FN(), None, 0), "progress cb")
+            self._cb = None
             return
 
         def tramp(p, _user):
-            q = p.contents
-            fn(dict(automata_done=q.automata_done, subsets_expanded=q.subsets_expanded,
-                    launches=q.launches, deferred=q.deferred))
-        self._cb = PROGRESS_FN(tramp)
-        self._check(self._L.ssw_set_progress_callback(self.h, self._cb, None, interval_ms), "progress cb")
+            fn(_progress_dict(p.contents))
+        cb = PROGRESS_FN(tramp)
+        self._check(self._L.ssw_set_progress_callback(self.h, cb, None, interval_ms), "progress cb")
+        self._cb = cb
 
 
 class Server:
     """Continuous-batching request loop on a context (ssw_server_*)."""
 
     def __init__(self, ctx: Context, mem_budget=0, small_n_max=20, levels_per_tick=32, max_large=64,
-                 want_witness=False):
+                 want_witness=False, small_workers=0):
         self._L = ctx._L
         self.ctx = ctx
-        cfg = ServerConfig(mem_budget, small_n_max, levels_per_tick, max_large, 1 if want_witness else 0)
+        cfg = ServerConfig(mem_budget, small_n_max, levels_per_tick, max_large, 1 if want_witness else 0,
+                           small_workers, 0)
         h = ctypes.c_void_p()
         rc = self._L.ssw_server_create(ctx.h, ctypes.byref(cfg), ctypes.byref(h))
         if rc:

@@ -34,7 +34,7 @@ extern "C" {
 #endif
 
 #define SSW_MAX_N 30
-#define SSW_ABI_VERSION 1
+#define SSW_ABI_VERSION 2
 
 enum ssw_status {
     SSW_OK = 0,
@@ -178,14 +178,19 @@ typedef struct ssw_progress {
     uint64_t subsets_expanded;   /* sum of E over finished automata */
     uint64_t launches;
     uint64_t deferred;           /* automata handed to the device-wide path */
+    uint64_t in_flight;          /* automata of the running solve not finished yet (0 between solves) */
 } ssw_progress;
 
 typedef void (*ssw_progress_fn)(const ssw_progress *p, void *user);
 
 int ssw_get_progress(ssw_context *ctx, ssw_progress *out);
-/* Calls fn from a library thread every interval_ms while a solve runs
-   (fn = NULL turns it off).  This is the hook a Notifications-style progress
-   reporter (GPGPU/Notifications/INotify.cs:7-10) subscribes to. */
+/* Calls fn from a library thread every interval_ms while a solve runs, and
+   once more after every solve that ended since the thread last looked (also
+   when set_progress_callback stops the thread), so even a solve shorter than
+   the interval is reported with its final counts.  fn = NULL turns it off;
+   the call returns after the thread has stopped, so fn is not called after
+   that.  This is the hook a Notifications-style progress reporter
+   (GPGPU/Notifications/INotify.cs:7-10) subscribes to. */
 int ssw_set_progress_callback(ssw_context *ctx, ssw_progress_fn fn, void *user,
                               int32_t interval_ms);
 
@@ -233,11 +238,16 @@ int ssw_generate_dotnet(int32_t seed, int32_t n, int64_t count, int32_t *a, int3
 typedef struct ssw_server ssw_server;
 
 typedef struct ssw_server_config {
-    uint64_t mem_budget;        /* bytes of device memory for admitted requests */
-    int32_t small_n_max;        /* n <= this runs in the batched engine */
-    int32_t levels_per_tick;    /* device-wide engine: levels per graph replay */
-    int32_t max_large;          /* device-wide engine: concurrent instances */
+    uint64_t mem_budget;        /* bytes of device memory for admitted large requests
+                                   (0 = half of the owning context's budget) */
+    int32_t small_n_max;        /* n <= this runs in the batched engine (at most 20) */
+    int32_t levels_per_tick;    /* device-wide engine: levels per graph replay (0 = 16) */
+    int32_t max_large;          /* device-wide engine: concurrent instances (0 = 256) */
     int32_t want_witness;       /* compute witnesses for every request */
+    int32_t small_workers;      /* batched rounds that may run at once, each on its
+                                   own streams (1..4, 0 = 2): a request arriving
+                                   while a round runs starts in the next idle one */
+    int32_t reserved;
 } ssw_server_config;
 
 typedef struct ssw_result {
@@ -260,7 +270,7 @@ typedef struct ssw_server_stats {
 
 int ssw_server_create(ssw_context *ctx, const ssw_server_config *cfg, ssw_server **out);
 /* Copies the request; returns at once with a ticket.  SSW_ERR_BUDGET when the
-   request alone can never fit the budget. */
+   request alone can never fit the budget (the same figure admission uses). */
 int ssw_server_submit(ssw_server *s, int32_t n, const int32_t *a, const int32_t *b,
                       uint64_t *ticket);
 /* SSW_OK and *out filled when done, SSW_PENDING otherwise.  A result is
@@ -268,9 +278,16 @@ int ssw_server_submit(ssw_server *s, int32_t n, const int32_t *a, const int32_t 
    later calls with it return SSW_ERR_NOTFOUND. */
 int ssw_server_poll(ssw_server *s, uint64_t ticket, ssw_result *out);
 /* As poll, blocking up to timeout_ms (< 0: no limit); SSW_ERR_TIMEOUT when
-   the result is not ready by then (the ticket stays valid). */
+   the result is not ready by then (the ticket stays valid).  When several
+   callers wait on one ticket, one gets the result and the others
+   SSW_ERR_NOTFOUND. */
 int ssw_server_wait(ssw_server *s, uint64_t ticket, int32_t timeout_ms, ssw_result *out);
+/* Percentiles over the last 2^20 completed requests. */
 int ssw_server_stats_get(ssw_server *s, ssw_server_stats *out);
+/* Stops taking submissions, finishes every request already submitted, wakes
+   the callers blocked in ssw_server_wait (they return their result, or
+   SSW_ERR_STATE) and returns once none is left inside the server's calls;
+   results not collected are dropped.  No call may use s after this one. */
 int ssw_server_destroy(ssw_server *s);
 
 #ifdef __cplusplus

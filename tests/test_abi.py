@@ -36,7 +36,7 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_abi_version():
-    assert ssw.lib().ssw_abi_version() == 1
+    assert ssw.lib().ssw_abi_version() == 2
 
 
 def test_struct_layout_matches_header(tmp_path):

@@ -20,15 +20,37 @@ PKG = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
 def test_progress_counters_and_callback(ctx):
     seen = []
     before = ctx.progress()
-    ctx.set_progress_callback(lambda p: seen.append(p["automata_done"]), 5)
+    assert before["in_flight"] == 0
+    ctx.set_progress_callback(seen.append, 5)
     n, delta, pos = W.uniform_n16(count=200_000)
     r = ctx.solve_packed(delta.reshape(-1), len(delta), n)
-    time.sleep(0.05)
-    ctx.set_progress_callback(None)
     after = ctx.progress()
+    time.sleep(0.05)  # ten intervals after the solve ended
+    ctx.set_progress_callback(None)  # returns after the library's thread stopped
     assert after["automata_done"] - before["automata_done"] == len(delta)
     assert after["subsets_expanded"] - before["subsets_expanded"] == int(r.expanded.sum())
-    assert seen and seen == sorted(seen)
+    assert after["in_flight"] == 0
+    # the solve is reported, however short against the interval, and the last
+    # report carries its final counts
+    assert seen
+    done = [p["automata_done"] for p in seen]
+    assert done == sorted(done)
+    assert done[-1] == after["automata_done"]
+    assert seen[-1]["in_flight"] == 0
+    assert all(0 <= p["in_flight"] <= len(delta) for p in seen)
+
+
+def test_progress_reports_a_solve_shorter_than_the_interval(ctx):
+    # a solve that starts and ends between two looks of the library's thread
+    # (interval 1 s, a few hundred microseconds of work) is still reported:
+    # at the next look, or by the last look made when the callback is turned off
+    seen = []
+    ctx.set_progress_callback(seen.append, 1000)
+    A, B = ssw.generate_uniform(12, 10, 1000)
+    ctx.compute(10, A, B)
+    final = ctx.progress()["automata_done"]
+    ctx.set_progress_callback(None)
+    assert seen and seen[-1]["automata_done"] == final and seen[-1]["in_flight"] == 0
 
 
 def test_driver_writes_csv(tmp_path):
