@@ -18,7 +18,7 @@ LIB = os.path.join(HERE, "libssw.so")
 ARCH = os.environ.get("SSW_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["ssw_batch.hip", "ssw_large.hip", "ssw_probe.hip", "ssw_runtime.cpp", "ssw_large.cpp", "ssw_gen.cpp",
-           "ssw_server.cpp", "ssw_tune.cpp"]
+           "ssw_server.cpp", "ssw_server_core.cpp", "ssw_tune.cpp"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
           "-I", os.path.join(ROOT, "include"), "-I", CSRC]

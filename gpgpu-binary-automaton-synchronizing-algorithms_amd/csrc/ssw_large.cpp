@@ -101,9 +101,21 @@ int LargeEngine::set_outputs(ssw_context &ctx, int32_t *length, uint8_t *sync, i
     return SSW_OK;
 }
 
-uint64_t LargeEngine::admit_bytes(int n, bool append, uint64_t ring_cap) {
+// The first frontier ring of an instance: every subset (plus the root's slot
+// in append mode, which keeps all levels) up to 2^16 entries; larger rings
+// grow between graph replays.
+uint64_t LargeEngine::first_ring_cap(int n, bool append) {
+    const uint64_t all = 1ull << n;
+    return append ? pow2_at_least(std::min<uint64_t>(all, 1ull << 16) + 1)
+                  : pow2_at_least(std::min<uint64_t>(all, 1ull << 16));
+}
+
+// Device bytes an instance holds when admitted: bitmap, first ring, tables
+// and (append mode) the level-start table.  Admission and the budget checks
+// made before it (the server's submit) use this one figure.
+uint64_t LargeEngine::admit_bytes(int n, bool append) {
     const uint64_t bm = std::max<uint64_t>(256, (1ull << n) / 8);
-    return bm + 4 * ring_cap + 8192 + (append ? 8ull * ((n - 1) * (n - 1) + 8) : 0);
+    return bm + 4 * first_ring_cap(n, append) + 8192 + (append ? 8ull * ((n - 1) * (n - 1) + 8) : 0);
 }
 
 int LargeEngine::admit(ssw_context &ctx, int64_t index, int n, const uint8_t *delta, const uint64_t *offset,
@@ -117,9 +129,8 @@ int LargeEngine::admit(ssw_context &ctx, int64_t index, int n, const uint8_t *de
         }
     if (slot < 0) return SSW_OK;  // engine full
     const uint64_t all = 1ull << n;
-    const uint64_t ring_cap = append ? pow2_at_least(std::min<uint64_t>(all, 1ull << 16) + 1)
-                                     : pow2_at_least(std::min<uint64_t>(all, 1ull << 16));
-    const uint64_t need = admit_bytes(n, append, ring_cap);
+    const uint64_t ring_cap = first_ring_cap(n, append);
+    const uint64_t need = admit_bytes(n, append);
     if (bytes_in_use + need > budget && bytes_in_use > 0) return SSW_OK;  // wait for memory
     if (need > budget) return ctx.fail(SSW_ERR_BUDGET, "n=%d needs %llu bytes, budget %llu", n,
                                        (unsigned long long)need, (unsigned long long)budget);
@@ -389,6 +400,9 @@ int LargeEngine::solve_all(ssw_context &ctx, const DeviceBatch &db, const uint8_
     hipError_t e;
     if ((e = fill_u32((uint32_t *)((uint8_t *)ctx.ctrs.p + kCtrViolation), 2, 0xFFFFFFFFu, ctx.stream)) !=
         hipSuccess)
+        return ctx.hip_fail(e, "fill");
+    // automata done in this solve <- 0
+    if ((e = fill_u32((uint32_t *)((uint8_t *)ctx.ctrs.p + kCtrProgress + 16), 2, 0u, ctx.stream)) != hipSuccess)
         return ctx.hip_fail(e, "fill");
     std::vector<uint32_t> idx(db.count);
     for (int64_t i = 0; i < db.count; ++i) idx[i] = (uint32_t)i;
