@@ -2,20 +2,20 @@
 
 The 1,000,000-automaton n = 16 batch (1 % Cerny C_16), the 200,000-automaton
 batch of uneven sizes (n = 12..24) and the large instances (C_24..C_30 plus
-64 random n = 30 automata) are each solved whole on the GPU, then checked
-through properties that do not depend on size and against the CPU oracle on
-samples:
+64 random n = 30 automata) are each solved whole on the GPU and compared,
+every automaton, with the CPU oracle (the SlimCPUUntweaked restatement, and
+its uint32 twin for n > 16): length, flag and expanded count, bit for bit.
+Beside that, properties that do not depend on size:
 
 * Cerny closed forms: C_n has length (n-1)^2 (SimpleAutomataTests.cs:14-35)
   and the search expands every subset but the n singletons and Q's last
   level, E = 2^n - n - 1;
 * repeatability: a second solve returns identical arrays;
-* the oracle (SlimCPUUntweaked restatement) on a seeded sample of each batch
-  plus every automaton of the sample's kind that the later launches take
-  (the widest frontiers: largest E, largest n);
 * the batched path against the plain one-automaton-per-launch path on a
   sample.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -24,6 +24,9 @@ import ssw
 import workloads as W
 
 pytestmark = pytest.mark.gpu
+
+
+THREADS = max(1, min(16, os.cpu_count() or 1))
 
 
 def _sample(count, k, seed):
@@ -53,18 +56,15 @@ def test_n16_lengths_in_range(n16):
     assert np.all(r.expanded >= 1)
 
 
-def test_n16_matches_oracle_on_sample(n16):
+def test_n16_matches_oracle_whole(n16):
+    # every one of the 1,000,000 automata, the 10,000 C_16 included
     n, delta, pos, r = n16
-    count = len(delta)
-    rest = np.setdiff1d(np.arange(count), pos)
-    widest = rest[np.argsort(r.expanded[rest])[-400:]]  # taken by the later launches
-    idx = np.union1d(_sample(count, 3000, 16), widest)
-    A = delta[idx, :n].astype(np.int32)
-    B = delta[idx, n:].astype(np.int32)
-    L, S, E, _ = O.slim(n, A, B, threads=8)
-    np.testing.assert_array_equal(r.length[idx], L)
-    np.testing.assert_array_equal(r.sync[idx], S)
-    np.testing.assert_array_equal(r.expanded[idx], E)
+    A = np.ascontiguousarray(delta[:, :n]).astype(np.int32)
+    B = np.ascontiguousarray(delta[:, n:]).astype(np.int32)
+    L, S, E, _ = O.slim(n, A, B, threads=THREADS)
+    bad = np.nonzero((r.length != L) | (r.sync != S) | (r.expanded != E))[0]
+    assert bad.size == 0, f"{bad.size} automata differ, first {bad[:10].tolist()}"
+    assert np.array_equal(r.length, L) and np.array_equal(r.sync, S) and np.array_equal(r.expanded, E)
 
 
 def test_n16_repeatable(ctx, n16):
@@ -84,7 +84,8 @@ def test_n16_batch_equals_plain_on_sample(ctx, n16):
     np.testing.assert_array_equal(p.expanded, r.expanded[idx])
 
 
-def test_uneven_matches_oracle_on_sample(ctx):
+def test_uneven_matches_oracle_whole(ctx):
+    # every one of the 200,000 automata of uneven sizes
     n_per, offs, delta = W.uneven_12_24()
     count = len(n_per)
     r = ctx.solve_packed(delta, count, 0, n_per, offs)
@@ -93,16 +94,16 @@ def test_uneven_matches_oracle_on_sample(ctx):
     np.testing.assert_array_equal(r2.expanded, r.expanded)
     L = r.length
     assert np.all((L == -1) | ((L >= 1) & (L <= (n_per.astype(np.int32) - 1) ** 2)))
-    idx = np.union1d(_sample(count, 2000, 18), np.nonzero(n_per >= 23)[0])
-    for n in np.unique(n_per[idx]):
-        sel = idx[n_per[idx] == n]
+    Lo = np.empty(count, np.int32)
+    So = np.empty(count, np.uint8)
+    Eo = np.empty(count, np.int64)
+    for n in np.unique(n_per):
+        sel = np.nonzero(n_per == n)[0]
         rows = offs[sel].astype(np.int64)[:, None] + np.arange(int(n))[None, :]
-        A = delta[rows].astype(np.int32)
-        B = delta[rows + int(n)].astype(np.int32)
-        Lo, So, Eo, _ = O.solve(int(n), A, B, threads=8)
-        np.testing.assert_array_equal(r.length[sel], Lo, err_msg=f"n={n}")
-        np.testing.assert_array_equal(r.sync[sel], So, err_msg=f"n={n}")
-        np.testing.assert_array_equal(r.expanded[sel], Eo, err_msg=f"n={n}")
+        Lo[sel], So[sel], Eo[sel], _ = O.solve(int(n), delta[rows].astype(np.int32),
+                                               delta[rows + int(n)].astype(np.int32), threads=THREADS)
+    bad = np.nonzero((r.length != Lo) | (r.sync != So) | (r.expanded != Eo))[0]
+    assert bad.size == 0, f"{bad.size} automata differ, first {bad[:10].tolist()} (n {n_per[bad[:10]].tolist()})"
 
 
 def test_large_instances(ctx):
