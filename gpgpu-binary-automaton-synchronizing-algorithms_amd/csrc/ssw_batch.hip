@@ -26,6 +26,8 @@
 //     its image is such a subset of the next level.
 #include "ssw_internal.h"
 
+#include <mutex>
+
 namespace ssw {
 
 // Explicit address spaces: LDS (3) and global (1).  With generic pointers the
@@ -1680,6 +1682,14 @@ __global__ __launch_bounds__(256) void batch_kernel(BatchParams P) {
     S.flush();
 }
 
+// Scratch of the allocation-free entry point (layout in ssw_internal.h): one
+// copy per GPU, held while the process runs.  In host code the name
+// `scratch` is a host-side stand-in, not a device address: the device
+// address of each GPU's copy comes from hipGetSymbolAddress
+// (noalloc_scratch below), and only that address is handed to kernels.
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+static_assert(4 * kQueueBytes <= kNoAllocListOff, "no-alloc work queues overlap the spill lists");
+
 // Clears `words` u32 words (kernel instead of a memset so the call can sit
 // inside a captured graph).
 __global__ void fill_u32_kernel(uint32_t *p, uint64_t words, uint32_t value) {
@@ -1779,6 +1789,34 @@ hipError_t reset_counters(uint32_t *deferred, unsigned long long *violation, uns
     hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, st, deferred, violation, call_done);
     return hipGetLastError();
 }
+
+// Device address of the allocation-free entry point's scratch on `device`,
+// which must be current: looked up once per device (at context set-up) and
+// kept, since every GPU holds its own copy of the array at its own address.
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_noalloc_call_mu[kMaxDevices];
+}  // namespace
+
+hipError_t noalloc_scratch(int device, void **out) {
+    *out = nullptr;
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[device] = p;
+    }
+    *out = g_scratch_ptr[device];
+    return hipSuccess;
+}
+
+// Calls of the allocation-free entry point on one GPU share its scratch: the
+// library serialises them per GPU with this lock (within one process).
+std::mutex &noalloc_call_lock(int device) { return g_noalloc_call_mu[device < 0 || device >= kMaxDevices ? 0 : device]; }
 
 hipError_t fill_u32(uint32_t *p, uint64_t words, uint32_t value, hipStream_t st) {
     if (words == 0) return hipSuccess;

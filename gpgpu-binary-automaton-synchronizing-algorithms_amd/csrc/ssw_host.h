@@ -58,8 +58,12 @@ struct ClassSpec {
     int plain_n = 0;
     hipStream_t stream = nullptr;     // null: the context's stream
     uint32_t *queues = nullptr;       // this class's device work queue
-    uint32_t *spill_queue[2] = {nullptr, nullptr};  // work queues and lists of the later launches
-    uint32_t *spill_list[2] = {nullptr, nullptr};   // of a spill chain (null: no spill launches)
+    uint32_t *spill_queue[3] = {nullptr, nullptr, nullptr};  // work queues and lists of the later
+    uint32_t *spill_list[3] = {nullptr, nullptr, nullptr};   // launches of a spill chain (null: none)
+    // allocation-free chain: the HBM rings of the last launch's groups come
+    // from this caller-provided memory and every earlier launch spills
+    uint8_t *static_slots = nullptr;
+    uint64_t static_slot_bytes = 0;
 };
 
 // launch passes of one class
@@ -214,6 +218,7 @@ struct ssw_context {
     int ensure(ssw::DevBuf &b, size_t bytes, const char *what);
     int get_slots(const ssw::SlotKey &key, uint64_t bytes, ssw::DevBuf **out, hipStream_t st);
     int launch_class(const ssw::DeviceBatch &db, const ssw::ClassSpec &cls, bool plain);
+    int launch_class_static(const ssw::DeviceBatch &db, const ssw::ClassSpec &cls);
     int launch_pass(const ssw::DeviceBatch &db, const ssw::ClassSpec &cls, const ssw_tuning &t, bool plain,
                     const ssw::PassIO &io);
     int solve_device(const ssw::DeviceBatch &db, const uint8_t *host_n, bool plain, int64_t *violation_out);
@@ -227,6 +232,12 @@ struct ssw_context {
     bool host_checked = false;
     bool host_no_length = false;
     int solve_entry(const ssw_batch *b, int32_t memory, int mode, int64_t *violation_out);
+    // the allocation-free entry point: device buffers, uniform n <= 16, no
+    // witness; every device buffer it uses is the context's (set up at
+    // creation) or this GPU's static scratch
+    int solve_noalloc(const ssw_batch *b, int64_t *violation_out);
+    void *noalloc_base = nullptr;       // device address of this GPU's scratch
+    hipError_t noalloc_status = hipSuccess;
     int solve_run(const ssw_batch *b, int32_t memory, int mode, int64_t *violation_out);
     int compute_i32(int32_t n, int64_t count, const int32_t *A, const int32_t *B, int32_t *length_out,
                     uint8_t *sync_out, uint8_t *witness_out, int32_t witness_stride, int64_t *expanded_out,

@@ -19,6 +19,22 @@ constexpr int kQueues = 64;
 constexpr int kQueueWords = 32;
 constexpr size_t kQueueBytes = kQueues * kQueueWords * 4 + 64;
 
+// Scratch of the allocation-free entry point (ssw_solve_batch_noalloc): one
+// file-scope device array per GPU (ssw_batch.hip) holding, for one uniform
+// batch of n <= 16 with at most kNoAllocMaxCount automata, the work queues of
+// its launch chain, three spill lists and the HBM rings of the last launch's
+// groups (every earlier launch hands a frontier that outgrows its LDS ring
+// to the next launch instead of moving it to HBM).
+constexpr uint64_t kNoAllocMaxCount = 1ull << 20;
+constexpr int kNoAllocMaxN = 16;
+constexpr uint64_t kNoAllocQueueOff = 0;                  // 4 work queues
+constexpr uint64_t kNoAllocListOff = 64ull << 10;         // 3 spill lists of kNoAllocMaxCount u32
+constexpr uint64_t kNoAllocSlotOff = kNoAllocListOff + 3ull * 4ull * kNoAllocMaxCount;
+constexpr uint64_t kNoAllocSlotBytes = 2ull * 65536;      // one 65536-entry u16 ring per group
+constexpr uint32_t kNoAllocSlots = 256;                   // groups of the last launch at most
+constexpr uint64_t kNoAllocScratchBytes = kNoAllocSlotOff + kNoAllocSlots * kNoAllocSlotBytes;
+constexpr uint32_t kNoAllocLastRing = 16384;              // LDS ring entries of the last launch
+
 // Result codes of one BFS level step.
 enum StepCode : int { kContinue = 0, kFound = 1, kNoSync = 2, kOverflow = 3, kSpilled = 4 };
 

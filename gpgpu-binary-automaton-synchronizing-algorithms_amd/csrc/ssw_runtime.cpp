@@ -35,9 +35,15 @@ hipError_t init_queues3(uint32_t *q0, uint32_t *q1, uint32_t *q2, int64_t count,
                         hipStream_t st);
 hipError_t reset_counters(uint32_t *deferred, unsigned long long *violation, unsigned long long *call_done,
                           hipStream_t st);
+hipError_t noalloc_scratch(int device, void **out);
+std::mutex &noalloc_call_lock(int device);
 }  // namespace ssw
 
 using namespace ssw;
+
+static_assert(SSW_NOALLOC_SCRATCH_BYTES == kNoAllocScratchBytes && SSW_NOALLOC_MAX_COUNT == kNoAllocMaxCount &&
+                  SSW_NOALLOC_MAX_N == kNoAllocMaxN,
+              "allocation-free limits in ssw.h");
 
 static thread_local std::string g_create_error;
 
@@ -114,6 +120,9 @@ int ssw_context::open(int dev, uint64_t mem_budget) {
     if (rc) return rc;
     HIP_TRY(hipMemsetAsync(ctrs.p, 0, kCtrBytes, stream), "hipMemset");
     HIP_TRY(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    // the allocation-free entry point's scratch on this device (its own
+    // address per GPU); a failure here is reported by that entry point only
+    noalloc_status = noalloc_scratch(dev, &noalloc_base);
     default_tuning();
     return SSW_OK;
 }
@@ -216,6 +225,7 @@ static inline uint32_t next_pow2(uint64_t x) {
 //   2. (narrow_bits only) the same ring with the full 2^n-bit bitmap,
 //   3. the larger ring (spill_arena_lds), byte tables.
 int ssw_context::launch_class(const DeviceBatch &db, const ClassSpec &cls, bool plain) {
+    if (cls.static_slots) return launch_class_static(db, cls);
     ssw_tuning t = plain ? ssw_tuning{256, 256, 1, 8192, 1, 0, 0, 0, 0} : tune[cls.nmax];
     const bool small = cls.lds && cls.nmax <= kMaxLdsN && (db.witness == nullptr || !plain);
     if (small && db.witness != nullptr) {
@@ -302,6 +312,73 @@ int ssw_context::launch_class(const DeviceBatch &db, const ClassSpec &cls, bool 
     t2.arena_lds = t.spill_arena_lds;
     t2.lane_entries = t.spill_lane_entries > 0 ? t.spill_lane_entries : t.lane_entries;
     return launch_pass(db, cls, t2, false, last);
+}
+
+// The allocation-free chain of a uniform class of n <= 16: the tuned first
+// launch (and with narrow bitmaps the full-bitmap one), then the larger ring
+// with byte tables, all of them handing a frontier that outgrows their LDS
+// ring to the next launch; the last launch (kNoAllocLastRing entries, one
+// wavefront per automaton) is the only one that may move a frontier to an
+// HBM ring, and its grid is capped so those rings fit the static scratch.
+int ssw_context::launch_class_static(const DeviceBatch &db, const ClassSpec &cls) {
+    ssw_tuning t = tune[cls.nmax];
+    if (t.spill_arena_lds <= 0) t.spill_arena_lds = 2048;
+    if (t.spill_lane_entries <= 0) t.spill_lane_entries = 4;
+    const int nb = (t.narrow_bits >= 8 && t.narrow_bits < cls.nmax) ? t.narrow_bits : 0;
+    hipStream_t st = cls.stream ? cls.stream : stream;
+    HIP_TRY(init_queues3(cls.queues, cls.spill_queue[0], nb ? cls.spill_queue[1] : nullptr, cls.items, 6u, st),
+            "init queues");
+    HIP_TRY(init_queues(cls.spill_queue[2], cls.items, st, true), "init queues");
+    PassIO first;
+    first.queue_ready = true;
+    first.kind = kPassSpillFrom;
+    first.out_list = cls.spill_list[0];
+    first.out_queue = cls.spill_queue[0];
+    first.narrow_bits = nb;
+    if (nb) {
+        first.skip_list = cls.spill_list[1];
+        first.skip_queue = cls.spill_queue[1];
+    }
+    int rc = launch_pass(db, cls, t, false, first);
+    if (rc) return rc;
+    PassIO wide;  // the larger ring, byte tables, spilling to the last launch
+    wide.kind = kPassSpillFrom;
+    wide.in_list = cls.spill_list[0];
+    wide.in_queue = cls.spill_queue[0];
+    wide.out_list = cls.spill_list[2];
+    wide.out_queue = cls.spill_queue[2];
+    wide.byte_tab = true;
+    if (nb) {
+        PassIO mid;
+        mid.kind = kPassSpillFrom;
+        mid.in_list = cls.spill_list[0];
+        mid.in_queue = cls.spill_queue[0];
+        mid.out_list = cls.spill_list[1];
+        mid.out_queue = cls.spill_queue[1];
+        ssw_tuning tm = t;
+        tm.chunk = std::min(t.chunk, 4);
+        rc = launch_pass(db, cls, tm, false, mid);
+        if (rc) return rc;
+        wide.in_list = cls.spill_list[1];
+        wide.in_queue = cls.spill_queue[1];
+    }
+    ssw_tuning t2 = t;
+    t2.group_lanes = 64;
+    t2.block_threads = 0;
+    t2.chunk = 1;
+    t2.arena_lds = t.spill_arena_lds;
+    t2.lane_entries = t.spill_lane_entries;
+    rc = launch_pass(db, cls, t2, false, wide);
+    if (rc) return rc;
+    PassIO last;
+    last.kind = kPassSpillTo;
+    last.in_list = cls.spill_list[2];
+    last.in_queue = cls.spill_queue[2];
+    last.byte_tab = true;
+    ssw_tuning t3 = t2;
+    t3.arena_lds = (int32_t)kNoAllocLastRing;
+    t3.lane_entries = 4;
+    return launch_pass(db, cls, t3, false, last);
 }
 
 int ssw_context::launch_pass(const DeviceBatch &db, const ClassSpec &cls, const ssw_tuning &t, bool plain,
@@ -460,13 +537,30 @@ int ssw_context::launch_pass(const DeviceBatch &db, const ClassSpec &cls, const 
         const uint64_t max_slots = std::max<uint64_t>(1, budget / 2 / P.slot_bytes);
         while ((uint64_t)grid * groups_per_block > max_slots && grid > 1) grid = grid / 2;
     }
-    const uint64_t slots = (uint64_t)grid * groups_per_block;
-    SlotKey key{G, block, P.lds_n, nmax, (int)want_wit, (int)arena_lds,
-                (plain ? 1 : 0) + 2 * variant + 32 * io.kind + 128 * io.narrow_bits};
-    DevBuf *sb = nullptr;
-    int rc = get_slots(key, slots * P.slot_bytes, &sb, st);
-    if (rc) return rc;
-    P.slot_base = (uint8_t *)sb->p;
+    if (cls.static_slots) {
+        // allocation-free chain: a spilling launch never leaves its LDS ring
+        // (no slot memory: every group aliases one unused slot); the last
+        // launch's groups get the HBM rings of the static scratch
+        if (!small || want_wit) return fail(SSW_ERR_INTERNAL, "allocation-free chain on the general kernel");
+        if (io.out_list) {
+            P.slot_bytes = 0;
+        } else {
+            const uint64_t max_groups = cls.static_slot_bytes / P.slot_bytes;
+            if (max_groups < (uint64_t)groups_per_block)
+                return fail(SSW_ERR_INTERNAL, "static scratch holds %llu rings of %llu bytes",
+                            (unsigned long long)max_groups, (unsigned long long)P.slot_bytes);
+            grid = (int)std::min<uint64_t>((uint64_t)grid, max_groups / (uint64_t)groups_per_block);
+        }
+        P.slot_base = cls.static_slots;
+    } else {
+        const uint64_t slots = (uint64_t)grid * groups_per_block;
+        SlotKey key{G, block, P.lds_n, nmax, (int)want_wit, (int)arena_lds,
+                    (plain ? 1 : 0) + 2 * variant + 32 * io.kind + 128 * io.narrow_bits};
+        DevBuf *sb = nullptr;
+        int rc = get_slots(key, slots * P.slot_bytes, &sb, st);
+        if (rc) return rc;
+        P.slot_base = (uint8_t *)sb->p;
+    }
     last_grid = grid;
     last_block = block;
     last_lds = lds;
@@ -850,6 +944,82 @@ int ssw_context::solve_run(const ssw_batch *b, int32_t memory, int mode, int64_t
     return rc;
 }
 
+// The allocation-free entry point.  Device buffers only, one uniform n <= 16,
+// no witness words, at most kNoAllocMaxCount automata.  Nothing is allocated:
+// the counters, stream and events are the context's (set up at creation),
+// the work queues, spill lists and the last launch's HBM rings are this GPU's
+// static scratch.  Every piece of the scratch a launch reads is written
+// earlier in the same call (queues by their set-up kernels, list entries by
+// the launch that appends them, a ring's window by the group that moves it
+// there), so what an earlier call left behind is never read.
+int ssw_context::solve_noalloc(const ssw_batch *b, int64_t *violation_out) {
+    if (!b || b->count < 0 || !b->delta || !b->length) return fail(SSW_ERR_ARG, "null batch field");
+    if (violation_out) *violation_out = -1;
+    if (b->uniform_n < 1 || b->uniform_n > kNoAllocMaxN || b->n_per)
+        return fail(SSW_ERR_ARG, "the allocation-free entry point takes one uniform n in [1,%d] (got uniform_n=%d%s)",
+                    kNoAllocMaxN, b->uniform_n, b->n_per ? " with n_per" : "");
+    if (b->witness) return fail(SSW_ERR_ARG, "the allocation-free entry point computes no witness words");
+    if ((uint64_t)b->count > kNoAllocMaxCount)
+        return fail(SSW_ERR_ARG, "%lld automata: the allocation-free entry point's scratch holds at most %llu",
+                    (long long)b->count, (unsigned long long)kNoAllocMaxCount);
+    if (b->count == 0) return SSW_OK;
+    if (!noalloc_base)
+        return fail(SSW_ERR_DEVICE, "no static scratch on device %d: %s", device, hipGetErrorString(noalloc_status));
+    std::lock_guard<std::mutex> gpu_lk(noalloc_call_lock(device));
+    HIP_TRY(hipSetDevice(device), "hipSetDevice");
+    DeviceBatch db;
+    db.count = b->count;
+    db.uniform_n = b->uniform_n;
+    db.bound = b->bound;
+    db.offset = b->offset;
+    db.delta = b->delta;
+    db.length = b->length;
+    db.sync = b->sync;
+    db.expanded = b->expanded;
+    uint8_t *base = (uint8_t *)noalloc_base;
+    ClassSpec cls;
+    cls.lds = true;
+    cls.nmax = b->uniform_n;
+    cls.items = b->count;
+    cls.stream = stream;
+    cls.queues = (uint32_t *)(base + kNoAllocQueueOff);
+    for (int j = 0; j < 3; ++j) {
+        cls.spill_queue[j] = (uint32_t *)(base + kNoAllocQueueOff + kQueueBytes * (uint64_t)(j + 1));
+        cls.spill_list[j] = (uint32_t *)(base + kNoAllocListOff) + (uint64_t)j * kNoAllocMaxCount;
+    }
+    cls.static_slots = base + kNoAllocSlotOff;
+    cls.static_slot_bytes = kNoAllocSlots * kNoAllocSlotBytes;
+    solve_begin((uint64_t)b->count);
+    launches = 0;
+    int rc = SSW_OK;
+    hipError_t e = reset_counters((uint32_t *)((uint8_t *)ctrs.p + kCtrDeferred),
+                                  (unsigned long long *)((uint8_t *)ctrs.p + kCtrViolation),
+                                  (unsigned long long *)((uint8_t *)ctrs.p + kCtrProgress) + 2, stream);
+    if (e == hipSuccess) e = hipEventRecord(ev0, stream);
+    if (e != hipSuccess) rc = hip_fail(e, "set-up");
+    if (rc == SSW_OK) rc = launch_class(db, cls, false);
+    alignas(8) uint8_t head[16];
+    if (rc == SSW_OK) {
+        e = hipEventRecord(ev1, stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(head, ctrs.p, sizeof head, hipMemcpyDeviceToHost, stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        float ms = 0;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev0, ev1);
+        if (e != hipSuccess) rc = hip_fail(e, "allocation-free solve");
+        last_ms = ms;
+        last_launches = launches;
+    }
+    solve_end();
+    if (rc) return rc;
+    uint32_t ndef = 0;
+    unsigned long long viol = ~0ull;
+    std::memcpy(&ndef, head + kCtrDeferred, 4);
+    std::memcpy(&viol, head + kCtrViolation, 8);
+    if (ndef) return fail(SSW_ERR_INTERNAL, "%u automata deferred on the allocation-free path", ndef);
+    if (violation_out) *violation_out = viol == ~0ull ? -1 : (int64_t)viol;
+    return SSW_OK;
+}
+
 // Packs automata [lo, hi) of the int32 arrays into the staging bytes,
 // checking every transition; returns the first bad automaton or -1.  The
 // check is a running unsigned maximum, so the copy loop vectorises; only a
@@ -1113,6 +1283,11 @@ int ssw_best_parallelism(ssw_context *ctx) { return ctx ? 1 : SSW_ERR_ARG; }
 int ssw_solve_batch(ssw_context *ctx, const ssw_batch *b, int32_t memory, int64_t *violation_out) {
     CTX_GUARD(ctx);
     return ctx->solve_entry(b, memory, kModeBatch, violation_out);
+}
+
+int ssw_solve_batch_noalloc(ssw_context *ctx, const ssw_batch *b, int64_t *violation_out) {
+    CTX_GUARD(ctx);
+    return ctx->solve_noalloc(b, violation_out);
 }
 
 int ssw_solve_plain(ssw_context *ctx, const ssw_batch *b, int32_t memory, int64_t *violation_out) {

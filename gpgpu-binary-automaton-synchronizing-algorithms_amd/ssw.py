@@ -39,7 +39,7 @@ EXPORTS = [
     "ssw_copy_to_host", "ssw_copy_on_device", "ssw_synchronize", "ssw_last_kernel_time", "ssw_generate_uniform",
     "ssw_generate_cerny", "ssw_generate_dotnet", "ssw_server_create", "ssw_server_submit",
     "ssw_server_poll", "ssw_server_wait", "ssw_server_stats_get", "ssw_server_destroy",
-    "ssw_measure_atomics",
+    "ssw_measure_atomics", "ssw_solve_batch_noalloc",
 ]
 
 
@@ -119,6 +119,7 @@ def lib():
             "ssw_verify": ([vp, i32, i64, vp, vp, i32, vp], ctypes.c_int),
             "ssw_best_parallelism": ([vp], ctypes.c_int),
             "ssw_solve_batch": ([vp, ctypes.POINTER(Batch), i32, vp], ctypes.c_int),
+            "ssw_solve_batch_noalloc": ([vp, ctypes.POINTER(Batch), vp], ctypes.c_int),
             "ssw_solve_plain": ([vp, ctypes.POINTER(Batch), i32, vp], ctypes.c_int),
             "ssw_solve_large": ([vp, ctypes.POINTER(Batch), i32, vp], ctypes.c_int),
             "ssw_get_tuning": ([vp, i32, ctypes.POINTER(Tuning)], ctypes.c_int),
@@ -357,6 +358,14 @@ class Context:
         fn = {"batch": self._L.ssw_solve_batch, "plain": self._L.ssw_solve_plain,
               "large": self._L.ssw_solve_large}[mode]
         self._check(fn(self.h, ctypes.byref(batch), SSW_DEVICE, ctypes.byref(viol)), f"solve_{mode}")
+        return viol.value
+
+    def solve_device_noalloc(self, batch: Batch):
+        """ssw_solve_batch_noalloc: device buffers, one n <= 16, no witness,
+        nothing allocated during the call."""
+        viol = ctypes.c_int64(-1)
+        self._check(self._L.ssw_solve_batch_noalloc(self.h, ctypes.byref(batch), ctypes.byref(viol)),
+                    "solve_batch_noalloc")
         return viol.value
 
     def synchronize(self):

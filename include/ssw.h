@@ -126,6 +126,25 @@ typedef struct ssw_batch {
 int ssw_solve_batch(ssw_context *ctx, const ssw_batch *b, int32_t memory,
                     int64_t *violation_out);
 
+/* Allocation-free variant of ssw_solve_batch with device buffers.  It has no
+   counterpart in the reference; it is for callers that cannot allocate device
+   memory during a call (tight serving loops, harnesses that forbid hipMalloc).
+   It calls no hipMalloc, hipFree or allocating library routine: every device
+   buffer it uses is the context's (set up by ssw_create) or one static scratch
+   array per GPU of SSW_NOALLOC_SCRATCH_BYTES (about 44 MiB, held while the
+   process runs), for the work queues, spill lists and HBM frontier rings of
+   the same kernels ssw_solve_batch runs.  Limits, refused with SSW_ERR_ARG:
+   every pointer of *b is a device pointer, one uniform_n in [1,
+   SSW_NOALLOC_MAX_N] (n_per NULL), no witness words (witness NULL), count
+   <= SSW_NOALLOC_MAX_COUNT.  Results equal ssw_solve_batch's.  Calls on one
+   GPU share its scratch; the library serialises them within a process, and a
+   call leaves the scratch holding its own state, which the next call
+   overwrites before reading. */
+#define SSW_NOALLOC_MAX_N 16
+#define SSW_NOALLOC_MAX_COUNT 1048576
+#define SSW_NOALLOC_SCRATCH_BYTES 46202880
+int ssw_solve_batch_noalloc(ssw_context *ctx, const ssw_batch *b, int64_t *violation_out);
+
 /* One kernel launch per automaton, each a 256-thread workgroup running the
    level loop: the plain GPU BFS the batched form is measured against (the
    role SlimGPUQueue.cs:32-208 plays in the reference). */
