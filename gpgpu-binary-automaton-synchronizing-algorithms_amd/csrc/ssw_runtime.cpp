@@ -570,6 +570,7 @@ int ssw_context::launch_pass(const DeviceBatch &db, const ClassSpec &cls, const 
         if (!io.in_queue && !io.queue_ready) HIP_TRY(init_queues(P.queues, P.count, st, false), "init queues");
         HIP_TRY(launch_batch(variant, G, P, grid, block, lds, st), "batch kernel launch");
         ++launches;
+        ++total_launches;
     } else {
         for (int64_t i = 0; i < db.count; ++i) {
             if (cls.host_n && cls.host_n[i] != cls.plain_n) continue;
@@ -577,9 +578,9 @@ int ssw_context::launch_pass(const DeviceBatch &db, const ClassSpec &cls, const 
             P.first_item = i;
             HIP_TRY(launch_batch(variant, G, P, 1, block, lds, st), "plain kernel launch");
             ++launches;
+            ++total_launches;
         }
     }
-    total_launches += launches;
     return SSW_OK;
 }
 
