@@ -448,12 +448,14 @@ __global__ void init_queues_kernel(uint32_t *q0, uint32_t *q1, uint32_t *q2, int
 // The per-solve counters: deferred count <- 0, smallest violating index <-
 // none, automata done in this solve <- 0.
 __global__ void reset_counters_kernel(uint32_t *deferred, unsigned long long *violation,
-                                      unsigned long long *call_done) {
+                                      unsigned long long *call_done, unsigned long long *launch_stats,
+                                      uint32_t stat_words) {
     if (threadIdx.x == 0) {
         *deferred = 0u;
         *violation = ~0ull;
         *call_done = 0ull;
     }
+    for (uint32_t k = threadIdx.x; k < stat_words; k += blockDim.x) launch_stats[k] = 0ull;
 }
 
 template <class Grp>
@@ -484,6 +486,8 @@ struct GroupSolver {
     // progress
     uint32_t done_local;
     uint64_t subsets_local;
+    uint64_t expanded_local;  // subsets actually expanded (this launch's statistics)
+    uint32_t found_end;       // end of the step that met a singleton
 
     static constexpr bool kNestedLoop = false;
     // LDS: one region of lds_group_bytes per group, then the group scratch
@@ -505,6 +509,8 @@ struct GroupSolver {
         taken = false;
         done_local = 0;
         subsets_local = 0;
+        expanded_local = 0;
+        found_end = 0;
     }
 
     __device__ __forceinline__ uint32_t aget(uint32_t p) const {
@@ -661,7 +667,10 @@ struct GroupSolver {
         if (na) aput(wr + off, a);
         if (nb) aput(wr + off + (na ? 1u : 0u), b);
         wr += total;
-        if (any_single) return kFound;
+        if (any_single) {
+            found_end = rd + G < hi ? rd + G : hi;
+            return kFound;
+        }
         rd += G;
         if (rd >= hi) {
             g.sync();
@@ -750,6 +759,7 @@ struct GroupSolver {
                 const uint32_t k = atomicAdd(P.deferred_count, 1u);
                 P.deferred_list[k] = (uint32_t)ai;
             }
+            expanded_local += rd;  // the device-wide path searches it again
         } else {
             const bool found = code == kFound;
             const int32_t L = found ? (int32_t)level + 1 : -1;
@@ -765,6 +775,7 @@ struct GroupSolver {
             if (!found && P.witness) clear_witness_row();
             ++done_local;
             subsets_local += (uint64_t)E;
+            expanded_local += found ? found_end : hi;
         }
         if (!lds_bm) {
             g.sync();
@@ -775,13 +786,21 @@ struct GroupSolver {
     }
 
     __device__ void flush() {
-        if (g.rank() == 0 && done_local) {
-            atomicAdd(&P.progress[0], (unsigned long long)done_local);
-            atomicAdd(&P.progress[1], (unsigned long long)subsets_local);
-            atomicAdd(&P.progress[2], (unsigned long long)done_local);
+        if (g.rank() == 0) {
+            if (done_local) {
+                atomicAdd(&P.progress[0], (unsigned long long)done_local);
+                atomicAdd(&P.progress[1], (unsigned long long)subsets_local);
+                atomicAdd(&P.progress[2], (unsigned long long)done_local);
+            }
+            if (P.launch_stats && (done_local || expanded_local)) {
+                atomicAdd(&P.launch_stats[0], (unsigned long long)done_local);
+                atomicAdd(&P.launch_stats[1], (unsigned long long)subsets_local);
+                atomicAdd(&P.launch_stats[2], (unsigned long long)expanded_local);
+            }
         }
         done_local = 0;
         subsets_local = 0;
+        expanded_local = 0;
     }
 };
 
@@ -846,6 +865,8 @@ struct SmallSolver {
     bool taken;
     uint32_t done_local;
     uint64_t subsets_local;
+    uint64_t expanded_local;  // subsets actually expanded (this launch's statistics)
+    uint32_t found_end;       // end of the step that met a singleton (ring position)
     // the next automaton, loaded while the current one is searched
     int64_t nx_ai;
     int32_t nx_n;
@@ -878,6 +899,8 @@ struct SmallSolver {
         taken = false;
         done_local = 0;
         subsets_local = 0;
+        expanded_local = 0;
+        found_end = 0;
         nx_ai = -1;
     }
     static __device__ __forceinline__ lds_u32 *l32(uint32_t a) { return (lds_u32 *)(size_t)a; }
@@ -995,9 +1018,9 @@ struct SmallSolver {
                 if (P.skip_list && k == (uint32_t)n) {
                     const uint32_t ia = g.or_low32(r < (uint32_t)n ? 1u << qa : 0u);
                     const uint32_t ib = g.or_low32(r < (uint32_t)n ? 1u << qb : 0u);
-                    if (ia == Q || ib == Q) return spill_to(P.skip_queue, P.skip_list);
+                    if (ia == Q || ib == Q) return spill_to(P.skip_queue, P.skip_list, 0u);
                 }
-                return spill_to(P.spill_queue, P.spill_list);
+                return spill_to(P.spill_queue, P.spill_list, 0u);
             }
             const auto label = [U, k](uint32_t x) {
                 const uint32_t below = (uint32_t)__popc(U & ((1u << x) - 1u));
@@ -1074,6 +1097,7 @@ struct SmallSolver {
         if ((na && is_single(root_a)) || (nb && is_single(root_b))) {
             hi = wr = 0;
             level = 0;
+            found_end = 0;
             return kFound;  // length 1, one subset expanded
         }
         hi = wr = (na ? 1u : 0u) + (nb ? 1u : 0u);
@@ -1106,7 +1130,10 @@ struct SmallSolver {
             if (in_lds) {
                 const uint32_t lim = rcap - 2u * G;
                 while (wr - rd <= lim) {
-                    if (step_body<true, 1>()) return kFound;
+                    if (step_body<true, 1>()) {
+                        found_end = min(rd + G, hi);
+                        return kFound;
+                    }
                     rd += G;
                     if (rd >= hi) {
                         ++level;
@@ -1229,7 +1256,10 @@ struct SmallSolver {
             *ctl(3u + c) = 0u;
         }
         wr = hi + added;
-        if (found) return kFound;
+        if (found) {
+            found_end = hi;
+            return kFound;
+        }
         if (added == 0u) return kNoSync;
         ++level;
         rd = hi;
@@ -1283,13 +1313,15 @@ struct SmallSolver {
         return sm != 0;
     }
 
-    // Hands the automaton to the second launch (larger LDS ring).
-    __device__ int spill() { return spill_to(P.spill_queue, P.spill_list); }
-    __device__ int spill_to(uint32_t *queue, uint32_t *list) {
+    // Hands the automaton to the next launch (larger LDS ring), which
+    // searches it again; `partial` = subsets expanded here so far.
+    __device__ int spill() { return spill_to(P.spill_queue, P.spill_list, rd + e0); }
+    __device__ int spill_to(uint32_t *queue, uint32_t *list, uint32_t partial) {
         if (g.rank() == 0) {
             const uint32_t k = atomicAdd(queue + seg * kQueueWords + 1, 1u);
             list[k] = (uint32_t)ai;
         }
+        expanded_local += partial;
         return kSpilled;
     }
 
@@ -1385,7 +1417,10 @@ struct SmallSolver {
 
     template <bool LDS, int TT>
     __device__ int step_in() {
-        if (step_body<LDS, TT>()) return kFound;
+        if (step_body<LDS, TT>()) {
+            found_end = min(rd + G * TT, hi);
+            return kFound;
+        }
         rd = uni<G>(rd + G * TT);
         if (rd >= hi) {
             if (G > 64 || !LDS)
@@ -1616,18 +1651,27 @@ struct SmallSolver {
         }
         ++done_local;
         subsets_local += (uint64_t)E;
+        expanded_local += (uint64_t)(found ? found_end : hi) + e0;
         lsync();
         if (done_local >= 256) flush();
     }
 
     __device__ void flush() {
-        if (g.rank() == 0 && done_local) {
-            atomicAdd(&P.progress[0], (unsigned long long)done_local);
-            atomicAdd(&P.progress[1], (unsigned long long)subsets_local);
-            atomicAdd(&P.progress[2], (unsigned long long)done_local);
+        if (g.rank() == 0) {
+            if (done_local) {
+                atomicAdd(&P.progress[0], (unsigned long long)done_local);
+                atomicAdd(&P.progress[1], (unsigned long long)subsets_local);
+                atomicAdd(&P.progress[2], (unsigned long long)done_local);
+            }
+            if (P.launch_stats && (done_local || expanded_local)) {
+                atomicAdd(&P.launch_stats[0], (unsigned long long)done_local);
+                atomicAdd(&P.launch_stats[1], (unsigned long long)subsets_local);
+                atomicAdd(&P.launch_stats[2], (unsigned long long)expanded_local);
+            }
         }
         done_local = 0;
         subsets_local = 0;
+        expanded_local = 0;
     }
 };
 
@@ -1785,8 +1829,9 @@ hipError_t init_queues(uint32_t *queues, int64_t count, hipStream_t st, bool emp
 }
 
 hipError_t reset_counters(uint32_t *deferred, unsigned long long *violation, unsigned long long *call_done,
-                          hipStream_t st) {
-    hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, st, deferred, violation, call_done);
+                          unsigned long long *launch_stats, uint32_t stat_words, hipStream_t st) {
+    hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, st, deferred, violation, call_done,
+                       launch_stats, stat_words);
     return hipGetLastError();
 }
 

@@ -85,6 +85,14 @@ struct PassIO {
 };
 constexpr int kClassQueues = 3;  // work queues per launch class: its own and two spill queues
 
+// One launch of the last batched solve, for ssw_last_launch_stats: its shape,
+// the HIP events recorded around it on its stream and its counters' slot.
+constexpr int kMaxLaunchRecords = 32;
+struct LaunchRecord {
+    int n = 0, variant = 0, G = 0, block = 0, grid = 0, lds = 0, ring = 0, T = 0, bitmap_bits = 0, pass = 0;
+    bool byte_tab = false, witness = false;
+};
+
 struct SlotKey {
     int G, block, lds_n, nmax, witness, arena_lds, plain;
     bool operator<(const SlotKey &o) const {
@@ -176,6 +184,12 @@ struct ssw_context {
     std::string err;
 
     ssw::DevBuf ctrs, queues, deferred, order, spill;
+    // per-launch statistics of the last batched solve: kMaxLaunchRecords
+    // triples of u64 counters (zeroed with the solve's counters), the events
+    // around each launch and its shape
+    ssw::DevBuf lstats;
+    hipEvent_t lev[2 * ssw::kMaxLaunchRecords] = {};
+    std::vector<ssw::LaunchRecord> records;
     ssw::DevBuf d_delta, d_nper, d_off, d_len, d_sync, d_exp, d_wit;
     std::map<ssw::SlotKey, ssw::DevBuf> slot_cache;
     std::map<ssw::SlotKey, uint64_t> slot_gen;   // call that last used each entry
@@ -243,6 +257,7 @@ struct ssw_context {
                     uint8_t *sync_out, uint8_t *witness_out, int32_t witness_stride, int64_t *expanded_out,
                     int32_t bound, int64_t *violation_out);
     int read_progress(ssw_progress *out);
+    int launch_stats(ssw_launch_stat *out, int32_t cap, int32_t *count);
     void start_progress_thread();
     void stop_progress_thread();
     int measure_atomics(int32_t kind, double *ops_per_s);

@@ -87,6 +87,8 @@ struct BatchParams {
     unsigned int *deferred_count;
     uint32_t *deferred_list;           // automata whose frontier outgrew the arena
     unsigned long long *progress;      // [0] automata done, [1] subsets expanded, [2] done in this solve
+    unsigned long long *launch_stats;  // null, or this launch's [0] automata finished, [1] their E
+                                       // summed, [2] subsets expanded (with partial searches handed on)
     uint8_t *slot_base;                // per-group global scratch
     uint64_t slot_bytes;
     uint64_t garena_off;               // u32 frontier arena beyond the LDS part

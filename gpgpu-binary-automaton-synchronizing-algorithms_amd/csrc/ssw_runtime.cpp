@@ -34,7 +34,7 @@ hipError_t init_queues(uint32_t *queues, int64_t count, hipStream_t st, bool emp
 hipError_t init_queues3(uint32_t *q0, uint32_t *q1, uint32_t *q2, int64_t count, uint32_t empty_mask,
                         hipStream_t st);
 hipError_t reset_counters(uint32_t *deferred, unsigned long long *violation, unsigned long long *call_done,
-                          hipStream_t st);
+                          unsigned long long *launch_stats, uint32_t stat_words, hipStream_t st);
 hipError_t noalloc_scratch(int device, void **out);
 std::mutex &noalloc_call_lock(int device);
 }  // namespace ssw
@@ -118,6 +118,10 @@ int ssw_context::open(int dev, uint64_t mem_budget) {
     if (rc) return rc;
     rc = ensure(queues, kQueueBytes, "work queues");
     if (rc) return rc;
+    rc = ensure(lstats, 24u * kMaxLaunchRecords, "launch statistics");
+    if (rc) return rc;
+    for (int k = 0; k < 2 * kMaxLaunchRecords; ++k) HIP_TRY(hipEventCreate(&lev[k]), "hipEventCreate");
+    HIP_TRY(hipMemsetAsync(lstats.p, 0, 24u * kMaxLaunchRecords, stream), "hipMemset");
     HIP_TRY(hipMemsetAsync(ctrs.p, 0, kCtrBytes, stream), "hipMemset");
     HIP_TRY(hipStreamSynchronize(stream), "hipStreamSynchronize");
     // the allocation-free entry point's scratch on this device (its own
@@ -130,9 +134,11 @@ int ssw_context::open(int dev, uint64_t mem_budget) {
 void ssw_context::close() {
     stop_progress_thread();
     if (device >= 0) hipSetDevice(device);
-    for (DevBuf *b : {&ctrs, &queues, &deferred, &order, &spill, &d_delta, &d_nper, &d_off, &d_len, &d_sync,
-                      &d_exp, &d_wit})
+    for (DevBuf *b : {&ctrs, &queues, &deferred, &order, &spill, &lstats, &d_delta, &d_nper, &d_off, &d_len,
+                      &d_sync, &d_exp, &d_wit})
         if (b->p) hipFree(b->p);
+    for (int k = 0; k < 2 * kMaxLaunchRecords; ++k)
+        if (lev[k]) hipEventDestroy(lev[k]);
     for (auto &kv : slot_cache)
         if (kv.second.p) hipFree(kv.second.p);
     slot_cache.clear();
@@ -568,7 +574,28 @@ int ssw_context::launch_pass(const DeviceBatch &db, const ClassSpec &cls, const 
 
     if (!plain) {
         if (!io.in_queue && !io.queue_ready) HIP_TRY(init_queues(P.queues, P.count, st, false), "init queues");
+        // the launch's statistics: its counters and the events around it on its stream
+        const int rec = records.size() < (size_t)kMaxLaunchRecords ? (int)records.size() : -1;
+        if (rec >= 0) {
+            LaunchRecord r;
+            r.n = nmax;
+            r.variant = variant;
+            r.G = G;
+            r.block = block;
+            r.grid = grid;
+            r.lds = (int)lds;
+            r.ring = (int)arena_lds;
+            r.T = small ? T : 1;
+            r.bitmap_bits = use_lds ? (int)(32u * bm_words) : 0;
+            r.pass = io.kind == kPassAll && !io.out_list ? 0 : (io.in_list ? 2 : 1);
+            r.byte_tab = byte_tab;
+            r.witness = want_wit;
+            records.push_back(r);
+            P.launch_stats = (unsigned long long *)lstats.p + 3 * rec;
+            HIP_TRY(hipEventRecord(lev[2 * rec], st), "hipEventRecord");
+        }
         HIP_TRY(launch_batch(variant, G, P, grid, block, lds, st), "batch kernel launch");
+        if (rec >= 0) HIP_TRY(hipEventRecord(lev[2 * rec + 1], st), "hipEventRecord");
         ++launches;
         ++total_launches;
     } else {
@@ -591,10 +618,13 @@ int ssw_context::solve_device(const DeviceBatch &db, const uint8_t *host_n, bool
     ++call_gen;
     int rc = ensure(deferred, 4ull * std::max<int64_t>(1, db.count), "deferred list");
     if (rc) return rc;
-    // violation <- UINT64_MAX, deferred count <- 0, done in this solve <- 0
+    // violation <- UINT64_MAX, deferred count <- 0, done in this solve <- 0,
+    // launch statistics <- 0
+    records.clear();
     HIP_TRY(reset_counters((uint32_t *)((uint8_t *)ctrs.p + kCtrDeferred),
                            (unsigned long long *)((uint8_t *)ctrs.p + kCtrViolation),
-                           (unsigned long long *)((uint8_t *)ctrs.p + kCtrProgress) + 2, stream),
+                           (unsigned long long *)((uint8_t *)ctrs.p + kCtrProgress) + 2,
+                           (unsigned long long *)lstats.p, 3u * kMaxLaunchRecords, stream),
             "reset counters");
     HIP_TRY(hipEventRecord(ev0, stream), "hipEventRecord");
 
@@ -993,9 +1023,11 @@ int ssw_context::solve_noalloc(const ssw_batch *b, int64_t *violation_out) {
     solve_begin((uint64_t)b->count);
     launches = 0;
     int rc = SSW_OK;
+    records.clear();
     hipError_t e = reset_counters((uint32_t *)((uint8_t *)ctrs.p + kCtrDeferred),
                                   (unsigned long long *)((uint8_t *)ctrs.p + kCtrViolation),
-                                  (unsigned long long *)((uint8_t *)ctrs.p + kCtrProgress) + 2, stream);
+                                  (unsigned long long *)((uint8_t *)ctrs.p + kCtrProgress) + 2,
+                                  (unsigned long long *)lstats.p, 3u * kMaxLaunchRecords, stream);
     if (e == hipSuccess) e = hipEventRecord(ev0, stream);
     if (e != hipSuccess) rc = hip_fail(e, "set-up");
     if (rc == SSW_OK) rc = launch_class(db, cls, false);
@@ -1145,6 +1177,47 @@ int ssw_context::compute_i32(int32_t n, int64_t count, const int32_t *A, const i
     }
     join_all();
     if (violation_out) *violation_out = viol;
+    return SSW_OK;
+}
+
+// ---------------------------------------------------------------------------
+// per-launch statistics of the last batched solve
+// ---------------------------------------------------------------------------
+
+int ssw_context::launch_stats(ssw_launch_stat *out, int32_t cap, int32_t *count) {
+    *count = (int32_t)records.size();
+    if (records.empty() || cap == 0) return SSW_OK;
+    HIP_TRY(hipSetDevice(device), "hipSetDevice");
+    unsigned long long c[3 * kMaxLaunchRecords];
+    HIP_TRY(hipMemcpyAsync(c, lstats.p, sizeof c, hipMemcpyDeviceToHost, stream), "D2H launch statistics");
+    HIP_TRY(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    for (size_t k = 0; k < records.size() && (int32_t)k < cap; ++k) {
+        const LaunchRecord &r = records[k];
+        ssw_launch_stat &o = out[k];
+        std::memset(&o, 0, sizeof o);
+        o.n = r.n;
+        o.variant = r.variant;
+        o.group_lanes = r.G;
+        o.block_threads = r.block;
+        o.grid = r.grid;
+        o.lds_bytes = r.lds;
+        o.ring_entries = r.ring;
+        o.lane_entries = r.T;
+        o.bitmap_bits = r.bitmap_bits;
+        o.pass = r.pass;
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, lev[2 * k], lev[2 * k + 1]), "hipEventElapsedTime");
+        o.ms = ms;
+        o.automata = c[3 * k];
+        o.subsets = c[3 * k + 1];
+        o.expanded = c[3 * k + 2];
+        const bool wave = r.G <= 64;
+        if (r.variant == 0)
+            std::snprintf(o.kernel, sizeof o.kernel, "GroupSolver<%s<%d>>", wave ? "WaveGroup" : "BlockGroup", r.G);
+        else
+            std::snprintf(o.kernel, sizeof o.kernel, "SmallSolver<%s<%d>,%d,%s%s>", wave ? "WaveGroup" : "BlockGroup",
+                          r.G, r.T, r.byte_tab ? "true" : "false", r.witness ? ",true" : "");
+    }
     return SSW_OK;
 }
 
@@ -1393,6 +1466,12 @@ int ssw_synchronize(ssw_context *ctx) {
     hipSetDevice(ctx->device);
     hipError_t e = hipStreamSynchronize(ctx->stream);
     return e == hipSuccess ? SSW_OK : ctx->hip_fail(e, "synchronize");
+}
+
+int ssw_last_launch_stats(ssw_context *ctx, ssw_launch_stat *out, int32_t cap, int32_t *count) {
+    CTX_GUARD(ctx);
+    if (!count || cap < 0 || (cap > 0 && !out)) return ctx->fail(SSW_ERR_ARG, "null argument");
+    return ctx->launch_stats(out, cap, count);
 }
 
 int ssw_last_kernel_time(ssw_context *ctx, double *ms, int64_t *launches) {

@@ -39,7 +39,7 @@ EXPORTS = [
     "ssw_copy_to_host", "ssw_copy_on_device", "ssw_synchronize", "ssw_last_kernel_time", "ssw_generate_uniform",
     "ssw_generate_cerny", "ssw_generate_dotnet", "ssw_server_create", "ssw_server_submit",
     "ssw_server_poll", "ssw_server_wait", "ssw_server_stats_get", "ssw_server_destroy",
-    "ssw_measure_atomics", "ssw_solve_batch_noalloc",
+    "ssw_measure_atomics", "ssw_solve_batch_noalloc", "ssw_last_launch_stats",
 ]
 
 
@@ -97,6 +97,14 @@ class ServerStats(ctypes.Structure):
                 ("subsets_expanded", ctypes.c_uint64)]
 
 
+class LaunchStat(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int32), ("variant", ctypes.c_int32), ("group_lanes", ctypes.c_int32),
+                ("block_threads", ctypes.c_int32), ("grid", ctypes.c_int32), ("lds_bytes", ctypes.c_int32),
+                ("ring_entries", ctypes.c_int32), ("lane_entries", ctypes.c_int32), ("bitmap_bits", ctypes.c_int32),
+                ("pass_", ctypes.c_int32), ("ms", ctypes.c_double), ("automata", ctypes.c_uint64),
+                ("subsets", ctypes.c_uint64), ("expanded", ctypes.c_uint64), ("kernel", ctypes.c_char * 64)]
+
+
 PROGRESS_FN = ctypes.CFUNCTYPE(None, ctypes.POINTER(Progress), ctypes.c_void_p)
 
 _lib = None
@@ -146,6 +154,7 @@ def lib():
             "ssw_server_stats_get": ([vp, ctypes.POINTER(ServerStats)], ctypes.c_int),
             "ssw_server_destroy": ([vp], ctypes.c_int),
             "ssw_measure_atomics": ([vp, i32, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+            "ssw_last_launch_stats": ([vp, ctypes.POINTER(LaunchStat), i32, ctypes.POINTER(i32)], ctypes.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -273,13 +282,13 @@ class Context:
             raise SswError(rc, f"{what}: {self._L.ssw_last_error(self.h).decode()}")
 
     # -- IComputable-shaped entry points (host int32 arrays, one n) --------
-    def compute(self, n, A, B, witness=False, bound=-1):
+    def compute(self, n, A, B, witness=False, bound=-1, want_sync=True, want_expanded=True):
         A = np.ascontiguousarray(A, np.int32).reshape(-1, n)
         B = np.ascontiguousarray(B, np.int32).reshape(-1, n)
         count = A.shape[0]
         length = np.empty(count, np.int32)
-        sync = np.empty(count, np.uint8)
-        expanded = np.empty(count, np.int64)
+        sync = np.empty(count, np.uint8) if want_sync else None
+        expanded = np.empty(count, np.int64) if want_expanded else None
         stride = witness_stride(n)
         wit = np.zeros((count, stride), np.uint8) if witness else None
         viol = ctypes.c_int64(-1)
@@ -378,6 +387,19 @@ class Context:
         self._check(self._L.ssw_measure_atomics(self.h, {"lds": 0, "hbm": 1}[kind], ctypes.byref(ops)),
                     "measure_atomics")
         return ops.value
+
+    def last_launch_stats(self):
+        """Per launch of the last batched solve: shape, HIP-event duration on
+        its stream, automata finished, their E and subsets expanded."""
+        buf = (LaunchStat * 32)()
+        cnt = ctypes.c_int32(0)
+        self._check(self._L.ssw_last_launch_stats(self.h, buf, 32, ctypes.byref(cnt)), "launch stats")
+        out = []
+        for k in range(min(cnt.value, 32)):
+            x = buf[k]
+            out.append({f: (getattr(x, f).decode() if f == "kernel" else getattr(x, f))
+                        for f, _ in LaunchStat._fields_ if f != "pass_"} | {"pass": x.pass_})
+        return out
 
     def last_kernel_time(self):
         ms = ctypes.c_double()

@@ -227,6 +227,36 @@ int ssw_last_kernel_time(ssw_context *ctx, double *ms, int64_t *launches);
 
 /* ---- measurement --------------------------------------------------------- */
 
+/* One kernel launch of the last batched solve on the context (no counterpart
+   in the reference; the per-launch measurement behind the roofline report).
+   Launches of one solve come in class order; within a class, a spill chain's
+   launches in order. */
+typedef struct ssw_launch_stat {
+    int32_t n;                /* largest n of the launch's class */
+    int32_t variant;          /* 0 = general kernel (HBM bitmaps, witnesses), > 0 = n <= 16 kernel */
+    int32_t group_lanes;      /* threads per automaton */
+    int32_t block_threads;    /* workgroup size */
+    int32_t grid;             /* workgroups */
+    int32_t lds_bytes;        /* LDS per workgroup */
+    int32_t ring_entries;     /* LDS frontier ring (arena) entries per automaton */
+    int32_t lane_entries;     /* frontier entries per lane per level step (at most) */
+    int32_t bitmap_bits;      /* LDS visited bitmap per automaton (0: in HBM) */
+    int32_t pass;             /* 0 the class's only launch, 1 first launch of a spill chain,
+                                 2 a later launch of the chain */
+    double ms;                /* time between HIP events recorded around the launch on its stream */
+    uint64_t automata;        /* automata this launch finished */
+    uint64_t subsets;         /* their canonical expanded counts E, summed */
+    uint64_t expanded;        /* subsets this launch expanded -- two test-and-sets each --
+                                 including the partial searches it handed to a later launch */
+    char kernel[64];          /* e.g. "SmallSolver<WaveGroup<64>,4,true>" */
+} ssw_launch_stat;
+
+/* Fills out[0 .. min(cap, *count)) and sets *count to the number of launches
+   of the last batched solve (ssw_solve_batch, ssw_compute, ssw_verify,
+   ssw_solve_batch_noalloc; at most 32 are recorded; the plain and device-wide
+   entry points record none). */
+int ssw_last_launch_stats(ssw_context *ctx, ssw_launch_stat *out, int32_t cap, int32_t *count);
+
 /* Test-and-set ceiling of the device (no counterpart in the reference; for the
    roofline report beside the solvers' own rates): every wavefront of the GPU
    issues atomicOr-with-return at random bits, either of an 8 KiB LDS bitmap

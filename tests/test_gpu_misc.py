@@ -236,3 +236,31 @@ def test_server_tickets_and_timeouts(ctx):
             srv.submit(31, np.zeros(31, np.int32), np.zeros(31, np.int32))
         with pytest.raises(ssw.SswError):
             srv.submit(5, np.array([0, 1, 2, 3, 5], np.int32), np.zeros(5, np.int32))
+
+
+def test_last_launch_stats(ctx):
+    # the launches of one solve account for every automaton once and for the
+    # batch's expanded counts; each has an event-timed duration.  The shared
+    # context runs the default tuning (one launch for n = 16); a context with
+    # the committed table runs the tuned three-launch spill chain.
+    n, delta, pos = W.uniform_n16(seed=99, count=100_000)
+    with ssw.Context(0) as tuned:
+        tuned.load_tuning(os.path.join(PKG, "tuning_gfx950.json"))
+        for c, chain in ((ctx, False), (tuned, True)):
+            r = c.solve_packed(delta.reshape(-1), len(delta), n)
+            st = c.last_launch_stats()
+            assert len(st) >= (3 if chain else 1)
+            assert sum(x["automata"] for x in st) == len(delta)
+            assert sum(x["subsets"] for x in st) == int(r.expanded.sum())
+            assert all(x["ms"] > 0 and x["grid"] > 0 and x["kernel"].startswith("SmallSolver") for x in st)
+            # subsets actually expanded: the finished searches' E less the
+            # unexpanded tails of their last levels, plus the partial
+            # searches handed to a later launch
+            assert 0.5 * int(r.expanded.sum()) < sum(x["expanded"] for x in st) < 2 * int(r.expanded.sum())
+            if chain:
+                # the Cerny automata (a permutation letter) run in the last launch
+                assert st[-1]["automata"] >= len(pos) and st[-1]["subsets"] >= len(pos) * (2 ** n - n - 1)
+                assert [x["pass"] for x in st] == [1] + [2] * (len(st) - 1)
+        # the plain path records none
+        tuned.solve_packed(delta[:10].reshape(-1), 10, n, mode="plain")
+        assert tuned.last_launch_stats() == []
