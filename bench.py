@@ -348,8 +348,9 @@ def bench_serving_sweep(ctx, rates):
             "plain_automata_per_s": plain_rate, "plain_s": plain_dt,
             "plain_equals_served": bool(np.array_equal(pr.length, [r["length"] for r in res1])),
             "served_vs_plain": best["automata_per_s"] / plain_rate,
-            "at_highest_load_one_small_worker": {k: one[k] for k in ("automata_per_s", "p99_ms", "small_n_p99_ms")},
-            "at_highest_load_two_small_workers": {k: rows[-1][k] for k in ("automata_per_s", "p99_ms",
+            "at_highest_load_one_small_worker": {k: one[k] for k in ("automata_per_s", "p99_ms", "random_p99_ms",
+                                                                     "small_n_p99_ms")},
+            "at_highest_load_two_small_workers": {k: rows[-1][k] for k in ("automata_per_s", "p99_ms", "random_p99_ms",
                                                                              "small_n_p99_ms")}}
 
 

@@ -133,8 +133,7 @@ struct LargeEngine {
     int grid = 1024;
     int tick_levels = 32;
     uint64_t bytes_in_use = 0;
-    hipGraphExec_t graph = nullptr;
-    int graph_levels = 0;
+    std::map<int, hipGraphExec_t> graphs;  // captured [step, plan] chains by level count
 
     LargeState *dstate() const;
     LargeInstance *dinst() const;

@@ -48,9 +48,8 @@ LargeInstance *LargeEngine::dinst() const {
 void LargeEngine::release() {
     for (auto &s : slots) free_slot_buffers(s);
     slots.clear();
-    if (graph) hipGraphExecDestroy(graph);
-    graph = nullptr;
-    graph_levels = 0;
+    for (auto &kv : graphs) hipGraphExecDestroy(kv.second);
+    graphs.clear();
     if (state.p) hipFree(state.p);
     state = DevBuf();
 }
@@ -179,10 +178,10 @@ int LargeEngine::admit(ssw_context &ctx, int64_t index, int n, const uint8_t *de
     return SSW_OK;
 }
 
+// The captured chain of `levels` [large_step, large_plan] pairs, built once
+// per level count and kept.
 int LargeEngine::build_graph(ssw_context &ctx, int levels) {
-    if (graph && graph_levels == levels) return SSW_OK;
-    if (graph) hipGraphExecDestroy(graph);
-    graph = nullptr;
+    if (graphs.count(levels)) return SSW_OK;
     hipGraph_t g = nullptr;
     LG_TRY(hipStreamBeginCapture(ctx.stream, hipStreamCaptureModeThreadLocal), "begin capture");
     for (int k = 0; k < levels; ++k) {
@@ -195,13 +194,11 @@ int LargeEngine::build_graph(ssw_context &ctx, int levels) {
         }
     }
     LG_TRY(hipStreamEndCapture(ctx.stream, &g), "end capture");
-    hipError_t e = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
+    hipGraphExec_t exec = nullptr;
+    hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
     hipGraphDestroy(g);
-    if (e != hipSuccess) {
-        graph = nullptr;
-        return ctx.hip_fail(e, "graph instantiate");
-    }
-    graph_levels = levels;
+    if (e != hipSuccess) return ctx.hip_fail(e, "graph instantiate");
+    graphs[levels] = exec;
     return SSW_OK;
 }
 
@@ -210,7 +207,7 @@ int LargeEngine::tick(ssw_context &ctx, int levels) {
     int rc = build_graph(ctx, levels);
     if (rc) return rc;
     LG_TRY(large_launch_plan(dstate(), dinst(), 0, ctx.stream), "plan");
-    LG_TRY(hipGraphLaunch(graph, ctx.stream), "graph launch");
+    LG_TRY(hipGraphLaunch(graphs[levels], ctx.stream), "graph launch");
     ctx.total_launches += 2 * levels;
     return read_back(ctx);
 }

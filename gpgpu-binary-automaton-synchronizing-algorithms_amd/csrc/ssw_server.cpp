@@ -11,6 +11,7 @@
 #include "ssw_server_core.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <memory>
 
@@ -48,7 +49,14 @@ struct HipSmall final : SmallBackend {
 // context from setup() to teardown().
 struct HipLarge final : LargeBackend {
     ssw_context *ctx = nullptr;
-    int levels = 16;
+    int levels = 16;  // most levels per graph replay
+    // levels of the next replay: halved while a replay takes longer than
+    // kTickTarget and doubled (up to `levels`) while it takes less than a
+    // quarter of it, so a small-request round queued behind a replay on a
+    // shared hardware queue waits about kTickTarget at most even when the
+    // running instances' levels are wide
+    int cur = 16;
+    static constexpr double kTickTarget = 2e-3;
     std::unique_lock<std::mutex> hold;
     uint8_t *d_delta = nullptr, *d_wit = nullptr;
     std::vector<uint8_t> hwit;
@@ -66,6 +74,7 @@ struct HipLarge final : LargeBackend {
             return SSW_ERR_DEVICE;
         }
         E.tick_levels = std::max(1, levels);
+        cur = E.tick_levels;
         return SSW_OK;
     }
     void teardown() override {
@@ -101,9 +110,18 @@ struct HipLarge final : LargeBackend {
         return rc;
     }
     int tick(std::string *err) override {
-        const int rc = ctx->large.tick(*ctx, ctx->large.tick_levels);
-        if (rc) *err = ctx->err;
-        return rc;
+        const auto t0 = std::chrono::steady_clock::now();
+        const int rc = ctx->large.tick(*ctx, cur);
+        if (rc) {
+            *err = ctx->err;
+            return rc;
+        }
+        const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (dt > kTickTarget && cur > 1)
+            cur = std::max(1, cur / 2);
+        else if (dt < kTickTarget / 4 && cur < ctx->large.tick_levels)
+            cur = std::min(ctx->large.tick_levels, 2 * cur);
+        return SSW_OK;
     }
     int harvest(bool witness, std::vector<Finished> *out, std::string *err) override {
         LargeEngine &E = ctx->large;

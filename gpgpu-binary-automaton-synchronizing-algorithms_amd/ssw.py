@@ -458,7 +458,7 @@ class Context:
 class Server:
     """Continuous-batching request loop on a context (ssw_server_*)."""
 
-    def __init__(self, ctx: Context, mem_budget=0, small_n_max=20, levels_per_tick=32, max_large=64,
+    def __init__(self, ctx: Context, mem_budget=0, small_n_max=20, levels_per_tick=32, max_large=0,
                  want_witness=False, small_workers=0):
         self._L = ctx._L
         self.ctx = ctx
