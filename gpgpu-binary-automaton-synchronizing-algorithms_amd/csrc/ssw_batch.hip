@@ -1070,6 +1070,22 @@ struct SmallSolver {
             }
         }
         lsync();
+        if constexpr (G == 64 && !WIT) {
+            // Verify mode: lane r applies the word of length 6 whose letter k
+            // is b when bit k of r is set (the 64 words one wavefront holds;
+            // SuperSlimGPUBreakthrough.cs:183-256 does the same with one
+            // thread per word); a singleton among the results proves L <= 6,
+            // which no bound >= 6 can be below, so the search is skipped
+            if (P.prefilter && (P.bound >= 0 ? P.bound : (n - 1) * (n - 1)) >= 6) {
+                uint32_t cur = Q;
+#pragma unroll
+                for (uint32_t k = 0; k < 6u; ++k) {
+                    const uint32_t im = image(cur);
+                    cur = ((r >> k) & 1u) ? (im >> 16) : (im & 0xFFFFu);
+                }
+                if (g.any(is_single(cur))) return kPrefiltered;
+            }
+        }
         // The root level is expanded here: level 1 = the distinct subsets
         // among a(Q), b(Q) other than Q (a permutation letter maps Q to
         // itself) seeds the ring, and Q counts once in E.  With no new
@@ -1631,6 +1647,12 @@ struct SmallSolver {
     __device__ void finish(int code) {
         if (code == kSpilled) {
             lsync();
+            return;
+        }
+        if (code == kPrefiltered) {
+            ++done_local;
+            lsync();
+            if (done_local >= 256) flush();
             return;
         }
         const bool found = code == kFound;

@@ -244,6 +244,8 @@ struct ssw_context {
     // lengths not wanted by the caller (Verify)
     bool host_checked = false;
     bool host_no_length = false;
+    // ssw_verify tries the 64 words of length 6 before each search
+    bool verify_prefilter = false;
     int solve_entry(const ssw_batch *b, int32_t memory, int mode, int64_t *violation_out);
     // the allocation-free entry point: device buffers, uniform n <= 16, no
     // witness; every device buffer it uses is the context's (set up at

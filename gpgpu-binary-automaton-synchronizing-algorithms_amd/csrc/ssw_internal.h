@@ -36,7 +36,7 @@ constexpr uint64_t kNoAllocScratchBytes = kNoAllocSlotOff + kNoAllocSlots * kNoA
 constexpr uint32_t kNoAllocLastRing = 16384;              // LDS ring entries of the last launch
 
 // Result codes of one BFS level step.
-enum StepCode : int { kContinue = 0, kFound = 1, kNoSync = 2, kOverflow = 3, kSpilled = 4 };
+enum StepCode : int { kContinue = 0, kFound = 1, kNoSync = 2, kOverflow = 3, kSpilled = 4, kPrefiltered = 5 };
 
 // LDS layout of the n <= 16 path, per workgroup: the visited bitmaps of all
 // groups first, each aligned to its size (a power of two, at least 256
@@ -128,6 +128,11 @@ struct BatchParams {
     // U = a(Q) | b(Q) takes the lowest labels and uses a visited bitmap of
     // 2^narrow_bits bits; automata with |U| > narrow_bits are spilled at start
     int32_t narrow_bits;
+    // Verify mode: before its search, each automaton tries the 64 words of
+    // length 6 from Q, one per lane (one wavefront per automaton); one that
+    // synchronizes proves L <= 6, within any bound >= 6, and the search is
+    // skipped (its length, flag and expanded count are not written)
+    uint32_t prefilter;
 };
 
 // Device-wide path: every instance keeps its visited bitmap (2^n bits) and a

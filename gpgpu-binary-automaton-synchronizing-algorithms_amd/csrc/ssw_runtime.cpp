@@ -424,6 +424,8 @@ int ssw_context::launch_pass(const DeviceBatch &db, const ClassSpec &cls, const 
     P.skip_queue = io.skip_queue;
     P.skip_list = io.skip_list;
     P.narrow_bits = small ? io.narrow_bits : 0;
+    // only Verify reads nothing but the violation, so only it may skip searches
+    P.prefilter = (host_no_length && verify_prefilter) ? 1u : 0u;
     P.length = db.length;
     P.sync = db.sync;
     P.expanded = db.expanded;
@@ -1353,6 +1355,14 @@ int ssw_verify(ssw_context *ctx, int32_t n, int64_t count, const int32_t *delta_
 }
 
 int ssw_best_parallelism(ssw_context *ctx) { return ctx ? 1 : SSW_ERR_ARG; }
+
+int ssw_set_verify_prefilter(ssw_context *ctx, int32_t word_length) {
+    CTX_GUARD(ctx);
+    if (word_length != 0 && word_length != 6)
+        return ctx->fail(SSW_ERR_ARG, "verify pre-filter word length %d (0 or 6)", word_length);
+    ctx->verify_prefilter = word_length == 6;
+    return SSW_OK;
+}
 
 int ssw_solve_batch(ssw_context *ctx, const ssw_batch *b, int32_t memory, int64_t *violation_out) {
     CTX_GUARD(ctx);

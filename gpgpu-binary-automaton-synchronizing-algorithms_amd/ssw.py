@@ -39,7 +39,7 @@ EXPORTS = [
     "ssw_copy_to_host", "ssw_copy_on_device", "ssw_synchronize", "ssw_last_kernel_time", "ssw_generate_uniform",
     "ssw_generate_cerny", "ssw_generate_dotnet", "ssw_server_create", "ssw_server_submit",
     "ssw_server_poll", "ssw_server_wait", "ssw_server_stats_get", "ssw_server_destroy",
-    "ssw_measure_atomics", "ssw_solve_batch_noalloc", "ssw_last_launch_stats",
+    "ssw_measure_atomics", "ssw_solve_batch_noalloc", "ssw_last_launch_stats", "ssw_set_verify_prefilter",
 ]
 
 
@@ -126,6 +126,7 @@ def lib():
             "ssw_compute": ([vp, i32, i64, vp, vp, vp, vp, vp, i32, vp, i32, vp], ctypes.c_int),
             "ssw_verify": ([vp, i32, i64, vp, vp, i32, vp], ctypes.c_int),
             "ssw_best_parallelism": ([vp], ctypes.c_int),
+            "ssw_set_verify_prefilter": ([vp, i32], ctypes.c_int),
             "ssw_solve_batch": ([vp, ctypes.POINTER(Batch), i32, vp], ctypes.c_int),
             "ssw_solve_batch_noalloc": ([vp, ctypes.POINTER(Batch), vp], ctypes.c_int),
             "ssw_solve_plain": ([vp, ctypes.POINTER(Batch), i32, vp], ctypes.c_int),
@@ -304,6 +305,10 @@ class Context:
         rc = self._L.ssw_verify(self.h, n, A.shape[0], _ptr(A), _ptr(B), bound, ctypes.byref(viol))
         self._check(rc, "ssw_verify")
         return viol.value
+
+    def set_verify_prefilter(self, word_length):
+        """6: ssw_verify tries the 64 words of length 6 before each search; 0: off."""
+        self._check(self._L.ssw_set_verify_prefilter(self.h, word_length), "set_verify_prefilter")
 
     def best_parallelism(self):
         return self._L.ssw_best_parallelism(self.h)

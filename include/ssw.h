@@ -92,6 +92,15 @@ int ssw_verify(ssw_context *ctx, int32_t n, int64_t count,
                const int32_t *delta_a, const int32_t *delta_b,
                int32_t bound, int64_t *violation_out);
 
+/* Verify-mode short-word pre-filter (SuperSlimGPUBreakthrough.cs:183-256,
+   with SlimCPUSkipper.cs:85-86 skipping what it proves): word_length 6 makes
+   ssw_verify on this context try, before each search of an automaton run with
+   one wavefront, the 64 words of length 6 from Q, one per lane; a word that
+   synchronizes proves a length <= 6, within any bound >= 6, and that search is
+   skipped.  0 (the default) turns it off.  Only ssw_verify is affected (it
+   returns the violating index alone); results are the same either way. */
+int ssw_set_verify_prefilter(ssw_context *ctx, int32_t word_length);
+
 /* IComputable.GetBestParallelism (IComputable.cs:18): the number of
    concurrent solver streams worth using; 1 for this library. */
 int ssw_best_parallelism(ssw_context *ctx);

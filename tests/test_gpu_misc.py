@@ -264,3 +264,44 @@ def test_last_launch_stats(ctx):
         # the plain path records none
         tuned.solve_packed(delta[:10].reshape(-1), 10, n, mode="plain")
         assert tuned.last_launch_stats() == []
+
+
+@pytest.mark.parametrize("n", [6, 9, 13, 16])
+def test_verify_prefilter_same_violation(n):
+    # ssw_verify with the length-6 word pre-filter returns what it returns
+    # without it, and what the oracle says, for bounds above, at and below 6
+    # (below 6 a word of length <= 6 proves nothing and the filter stays off)
+    A, B = ssw.generate_uniform(70 + n, n, 20_000)
+    if n >= 3:
+        ca, cb = ssw.generate_cerny(n, 1)
+        A[::501] = ca
+        B[::501] = cb
+    L = O.slim(n, A, B, threads=8)[0]
+    with ssw.Context(0) as c:
+        c.load_tuning(os.path.join(PKG, "tuning_gfx950.json"))
+        for bound in (-1, 12, 7, 6, 5, 3, 1):
+            want = O.slim(n, A, B, bound=bound, threads=8)[3]
+            c.set_verify_prefilter(0)
+            off = c.verify(n, A, B, bound=bound)
+            c.set_verify_prefilter(6)
+            on = c.verify(n, A, B, bound=bound)
+            assert off == want and on == want, (n, bound, off, on, want)
+        # the filter runs: the searches it skips expand nothing (about a third
+        # of random automata have a reset word of length <= 6 for n >= 9)
+        spent = {}
+        for pf in (0, 6):
+            c.set_verify_prefilter(pf)
+            p0 = c.progress()
+            c.verify(n, A, B)
+            p1 = c.progress()
+            assert p1["automata_done"] - p0["automata_done"] == len(A)
+            spent[pf] = p1["subsets_expanded"] - p0["subsets_expanded"]
+        if c.get_tuning(n)["group_lanes"] == 64:  # the filter needs one wavefront per automaton
+            assert spent[6] < spent[0]
+        else:
+            assert spent[6] == spent[0]
+        c.set_verify_prefilter(6)
+        # Compute is unaffected by the option: lengths of every automaton
+        np.testing.assert_array_equal(c.compute(n, A, B).length, L)
+        with pytest.raises(ssw.SswError):
+            c.set_verify_prefilter(5)
