@@ -851,6 +851,7 @@ struct SmallSolver {
     glb_u16 *hist;   // WIT: subsets in discovery order, 65536 entries
     glb_u8 *goodp;   // WIT: 1 = the subset at this position of hist is on a shortest path
     glb_u32 *lvlp;   // WIT: start position of each level in hist
+    glb_u32 *choicep;  // WIT: bit x = letter a keeps good subset x on a shortest path
     uint32_t rmask, rcap;
     bool in_lds;
     bool narrow;  // relabelled states, visited bitmap of 2^narrow_bits bits
@@ -889,6 +890,7 @@ struct SmallSolver {
         hist = (glb_u16 *)(slot + P.hist_off);
         goodp = (glb_u8 *)(slot + P.good_off);
         lvlp = (glb_u32 *)(slot + P.lvl_off);
+        choicep = (glb_u32 *)(slot + P.gbitmap_off);
         rcap = P.lds_arena_cap;
         rmask = rcap - 1u;
         dum_a = stage_a + 4u * (g.rank() & (kDummies - 1u));
@@ -1514,11 +1516,14 @@ struct SmallSolver {
     // word the reference's back-tracking yields (CPU.cs:262-269).  Backward,
     // level j = L-1 .. 1: a subset of level j is good when an image of it is a
     // good subset of level j + 1 (level L - 1: an image is a singleton).  The
-    // LDS bitmap holds the good subsets of level j + 1 only: level j's flags
-    // are all computed (and stored per position) before the bitmap is cleared
-    // and refilled with level j's good subsets, so a mark never leaks across
-    // levels.  Forward: letter a whenever a(cur) is a good subset of the next
-    // level (a search of that level's positions; the bitmap for level 1).
+    // LDS bitmap accumulates the good subsets of levels L-1 .. j+1 and is never
+    // cleared on the way: an image of a level-j subset lies in a level <= j+1,
+    // so of those marks only level j+1's can match it, and level j's flags are
+    // all computed before level j's own marks are added.  Forward: letter a
+    // whenever a(cur) is a good subset of the next level (for level 1 the
+    // marks, where only level 1 can hold a(Q)).  A history still in the LDS ring is
+    // searched there; a history in HBM (the long searches) goes through
+    // witness_hbm.
     __device__ void witness(int32_t L) {
         constexpr uint32_t U = 4;  // entries per lane per round, loads issued together
         const uint32_t r = g.rank();
@@ -1528,11 +1533,11 @@ struct SmallSolver {
         if (w.starts_cached)
             for (uint32_t k = r; k <= (uint32_t)L; k += G) *l32(w.starts_a + 4u * k) = lvlp[k];
         lsync();
-        // a level of at most G entries (most of them) takes one entry per
-        // lane, and the bitmap is then cleared bit by bit from the marks it
-        // set (prev) instead of wholesale
-        bool prev_narrow = false;
-        uint32_t prev = 0;
+        if (!w.local) {
+            witness_hbm(L, w);
+            return;
+        }
+        // a level of at most G entries (most of them) takes one entry per lane
         for (int32_t j = L - 1; j >= 1; --j) {
             const uint32_t s = level_start(w, j), e = level_start(w, j + 1);
             const bool last = j == L - 1;
@@ -1544,20 +1549,10 @@ struct SmallSolver {
                 const bool on = p < e && (last ? (is_single(a) || is_single(b)) : (marked(a) || marked(b)));
                 if (p < e) set_flag(w, p, on);
                 lsync();
-                if (prev_narrow) {
-                    if (prev) __hip_atomic_fetch_and(l32(bm_a | ((prev >> 3) & 0x1FFCu)), ~(1u << (prev & 31u)),
-                                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                } else {
-                    clear_bitmap();
-                }
-                lsync();
                 if (on) or_wg(l32(bm_a | ((x >> 3) & 0x1FFCu)), 1u << (x & 31u));
-                prev = on ? x : 0u;
-                prev_narrow = true;
                 lsync();
                 continue;
             }
-            prev_narrow = false;
             const bool one_round = e - s <= U * G;
             uint32_t keep[U] = {};
             for (uint32_t base = s; base < e; base += U * G) {
@@ -1578,14 +1573,11 @@ struct SmallSolver {
                 }
             }
             lsync();
-            clear_bitmap();
-            lsync();
             if (one_round) {
 #pragma unroll
                 for (uint32_t u = 0; u < U; ++u)
                     if (keep[u]) or_wg(l32(bm_a | ((keep[u] >> 3) & 0x1FFCu)), 1u << (keep[u] & 31u));
             } else {
-                if (!w.local) wave_sync();  // this level's flags are visible to every lane
                 for (uint32_t base = s; base < e; base += U * G) {
                     uint32_t x[U], f[U];
 #pragma unroll
@@ -1601,7 +1593,6 @@ struct SmallSolver {
             }
             lsync();
         }
-        if (!w.local) wave_sync();  // every level's flags are visible to every lane
         glb_u8 *out = row();
         uint32_t cur = Q, acc = 0;
         for (int32_t k = 0; k < L; ++k) {
@@ -1613,6 +1604,7 @@ struct SmallSolver {
             } else if (k == 0) {
                 take_a = marked(a);  // the bitmap holds level 1's good subsets
             } else {
+                // a is taken when a(cur) is a flagged entry of level k + 1
                 const uint32_t s = level_start(w, k + 1), e = level_start(w, k + 2);
                 bool hit = false;
                 if (e - s <= G) {
@@ -1636,6 +1628,210 @@ struct SmallSolver {
             }
             if (!take_a) acc |= 1u << (k & 7);
             cur = take_a ? a : b;
+            if ((k & 7) == 7 || k == L - 1) {
+                if (r == 0 && (k >> 3) < P.witness_stride) out[k >> 3] = (uint8_t)acc;
+                acc = 0;
+            }
+        }
+        zero_row((L + 7) >> 3);
+    }
+
+    // One level of the backward pass for a history in HBM, a round of U * G
+    // entries at a time: good flags into the LDS keep area (in_lds) or into
+    // HBM, then this level's marks added.
+    __device__ void witness_level_hbm(uint32_t s, uint32_t e, bool last, uint32_t keep_a, bool in_lds) {
+        constexpr uint32_t U = 4, RW = U * G;
+        const uint32_t r = g.rank();
+        for (uint32_t base = s; base < e; base += RW) {
+            uint32_t x[U];
+#pragma unroll
+            for (uint32_t u = 0; u < U; ++u) {
+                const uint32_t p = base + u * G + r;
+                x[u] = p < e ? (uint32_t)hist[p] : 0u;
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < U; ++u) {
+                const uint32_t p = base + u * G + r;
+                const uint32_t im = image(x[u]);
+                const uint32_t a = im & 0xFFFFu, b = im >> 16;
+                const bool ga = last ? is_single(a) : marked(a);
+                const bool on = p < e && (ga || (last ? is_single(b) : marked(b)));
+                if (on && ga) choose(x[u]);
+                if (p < e) {
+                    if (in_lds)
+                        *l16(keep_a + 2u * (p - s)) = (uint16_t)(on ? x[u] : 0u);  // no subset is empty
+                    else
+                        goodp[p] = on ? 1u : 0u;
+                }
+            }
+        }
+        if (in_lds)
+            lsync();
+        else
+            wave_sync();  // the level's flags in HBM are visible to every lane
+        for (uint32_t base = s; base < e; base += RW) {
+            uint32_t x[U];
+#pragma unroll
+            for (uint32_t u = 0; u < U; ++u) {
+                const uint32_t p = base + u * G + r;
+                if (in_lds)
+                    x[u] = p < e ? (uint32_t)*l16(keep_a + 2u * (p - s)) : 0u;
+                else
+                    x[u] = (p < e && goodp[p]) ? (uint32_t)hist[p] : 0u;
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < U; ++u)
+                if (x[u]) or_wg(l32(bm_a | ((x[u] >> 3) & 0x1FFCu)), 1u << (x[u] & 31u));
+        }
+        lsync();
+    }
+    // bit x of the choice bitmap: letter a keeps good subset x on a shortest path
+    __device__ __forceinline__ void choose(uint32_t x) const {
+        __hip_atomic_fetch_or(choicep + (x >> 5), 1u << (x & 31u), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+
+    // The passes for a history in HBM with the good subsets as lists in LDS.
+    // Backward, j = L-1 .. 1: the good subsets of level j are appended to the
+    // list area (the LDS ring after the cached level starts, which the search
+    // no longer needs) in ballot order, followed by their count; level j's
+    // marks are then set from its list.  The first round of level j - 1 is
+    // loaded before level j's marks are set.  Forward: letter a when a(cur)
+    // is in the next level's list (read from the top of the area down: G_1,
+    // its count above it, then G_2 ...).  A shortest path's levels hold few
+    // good subsets (one each for the Cerny automata), so the lists are short
+    // and the passes write nothing to HBM.  Returns false when the lists
+    // outgrow the area.
+    __device__ bool witness_lists(int32_t L, const WitnessSpace &w) {
+        constexpr uint32_t U = 4, RW = U * G;
+        const uint32_t r = g.rank();
+        const uint32_t starts_bytes = w.starts_cached ? 4u * ((uint32_t)L + 1u) : 0u;
+        const uint32_t list_a = ring_a + ((starts_bytes + 3u) & ~3u);
+        const uint32_t cap = (2u * (rcap + 64u) - (list_a - ring_a)) / 2u;  // u16 entries
+        uint32_t top = 0;  // u16 entries in use (uniform)
+        uint32_t x[U];
+        bool have = false;  // x holds the first round of the level
+        for (int32_t j = L - 1; j >= 1; --j) {
+            const uint32_t s = level_start(w, j), e = level_start(w, j + 1);
+            const bool last = j == L - 1;
+            const uint32_t first = top;
+            for (uint32_t base = s; base < e; base += RW) {
+                if (!(have && base == s)) {
+#pragma unroll
+                    for (uint32_t u = 0; u < U; ++u) {
+                        const uint32_t p = base + u * G + r;
+                        x[u] = p < e ? (uint32_t)hist[p] : 0u;
+                    }
+                }
+#pragma unroll
+                for (uint32_t u = 0; u < U; ++u) {
+                    const uint32_t p = base + u * G + r;
+                    const uint32_t im = image(x[u]);
+                    const uint32_t a = im & 0xFFFFu, b = im >> 16;
+                    const bool on =
+                        p < e && (last ? (is_single(a) | is_single(b)) : (marked(a) | marked(b)));
+                    const uint64_t m = __ballot(on);
+                    const uint32_t cnt = (uint32_t)__popcll(m);
+                    if (top + cnt >= cap) return false;  // no room for them and the count
+                    if (on)
+                        *l16(list_a + 2u * __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                                   __builtin_amdgcn_mbcnt_lo((uint32_t)m, top))) =
+                            (uint16_t)x[u];
+                    top += cnt;
+                }
+            }
+            have = false;
+            if (j > 1) {
+                const uint32_t s2 = level_start(w, j - 1);
+#pragma unroll
+                for (uint32_t u = 0; u < U; ++u) {
+                    const uint32_t p = s2 + u * G + r;
+                    x[u] = p < s ? (uint32_t)hist[p] : 0u;
+                }
+                have = true;
+            }
+            if (r == 0) *l16(list_a + 2u * top) = (uint16_t)(top - first);
+            ++top;
+            lsync();
+            for (uint32_t q = first + r; q + 1u < top; q += G) {
+                const uint32_t y = *l16(list_a + 2u * q);
+                or_wg(l32(bm_a | ((y >> 3) & 0x1FFCu)), 1u << (y & 31u));
+            }
+            lsync();
+        }
+        glb_u8 *out = row();
+        uint32_t cur = Q, acc = 0, endp = top;
+        for (int32_t k = 0; k < L; ++k) {
+            const uint32_t im = image(cur);
+            const uint32_t a = im & 0xFFFFu, b = im >> 16;
+            bool take_a;
+            if (k == L - 1) {
+                take_a = is_single(a);
+            } else {
+                // G_{k+1}: its count at endp - 1, its entries below it
+                const uint32_t c = uni<G>((uint32_t)*l16(list_a + 2u * (endp - 1u)));
+                const uint32_t lo = endp - 1u - c;
+                bool mine = false;
+                for (uint32_t q = lo + r; q + 1u < endp; q += G) mine |= *l16(list_a + 2u * q) == a;
+                take_a = g.any(mine);
+                endp = lo;
+            }
+            if (!take_a) acc |= 1u << (k & 7);
+            cur = take_a ? a : b;
+            if ((k & 7) == 7 || k == L - 1) {
+                if (r == 0 && (k >> 3) < P.witness_stride) out[k >> 3] = (uint8_t)acc;
+                acc = 0;
+            }
+        }
+        zero_row((L + 7) >> 3);
+        return true;
+    }
+
+    // The passes for a history in HBM (a long search: its level chain is the
+    // cost, so round trips per level are what is cut).  First with the good
+    // subsets as lists (witness_lists); when those outgrow their room, again
+    // from a clean bitmap with choice bits.  Backward: the good subsets of a
+    // level are kept in the LDS ring area, which the search no longer needs
+    // (after the cached level starts), so its marks are set from there (no
+    // flags stored in HBM, waited for and read back); a level wider than that
+    // room stores its flags in HBM and reads them back.  Each good subset x
+    // also gets its letter: bit x of the choice bitmap (the group's HBM slot)
+    // is set when a(x) is good one level further (level L - 1: a(x) is a
+    // singleton), which is exactly the forward greedy's test at x, since
+    // every subset lies in one level.  Forward: the first letter from the
+    // marks of level 1, then the choice bits, copied into the LDS bitmap, one
+    // LDS read per letter (no search of a level's positions).
+    __device__ void witness_hbm(int32_t L, const WitnessSpace &w) {
+        if (witness_lists(L, w)) return;
+        const uint32_t r = g.rank();
+        lsync();
+        clear_bitmap();
+        lsync();
+        const uint32_t starts_bytes = w.starts_cached ? 4u * ((uint32_t)L + 1u) : 0u;
+        const uint32_t keep_a = ring_a + ((starts_bytes + 3u) & ~3u);
+        const uint32_t keep_cap = (2u * (rcap + 64u) - (keep_a - ring_a)) / 2u;  // u16 entries
+        for (int32_t j = L - 1; j >= 1; --j) {
+            const uint32_t s = level_start(w, j), e = level_start(w, j + 1);
+            witness_level_hbm(s, e, j == L - 1, keep_a, e - s <= keep_cap);
+        }
+        // the first letter from the marks of level 1, then the choice bits
+        // (visible to every lane once the wave's atomics have completed)
+        const uint32_t im0 = image(Q);
+        const bool first_a = L == 1 ? is_single(im0 & 0xFFFFu) : marked(im0 & 0xFFFFu);
+        wave_sync();
+        const uint32_t words = P.lds_bitmap_words;
+        for (uint32_t k = r; k < words; k += G) {
+            // read where the atomics were performed (past the vector L1)
+            *l32(bm_a + 4u * k) = __hip_atomic_load(choicep + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            choicep[k] = 0u;  // clean for the next automaton
+        }
+        lsync();
+        glb_u8 *out = row();
+        uint32_t cur = Q, acc = 0;
+        for (int32_t k = 0; k < L; ++k) {
+            const uint32_t im = k == 0 ? im0 : image(cur);
+            const bool take_a = k == 0 ? first_a : marked(cur);
+            if (!take_a) acc |= 1u << (k & 7);
+            cur = take_a ? (im & 0xFFFFu) : (im >> 16);
             if ((k & 7) == 7 || k == L - 1) {
                 if (r == 0 && (k >> 3) < P.witness_stride) out[k >> 3] = (uint8_t)acc;
                 acc = 0;
@@ -1698,7 +1894,7 @@ struct SmallSolver {
 };
 
 template <class Solver, class Grp>
-__global__ __launch_bounds__(256) void batch_kernel(BatchParams P) {
+__device__ __forceinline__ void batch_body(const BatchParams &P) {
     extern __shared__ __align__(16) uint8_t smem[];
     constexpr uint32_t G = Grp::kSize;
     const uint32_t groups_per_block = blockDim.x / G;
@@ -1746,6 +1942,11 @@ __global__ __launch_bounds__(256) void batch_kernel(BatchParams P) {
         }
     }
     S.flush();
+}
+
+template <class Solver, class Grp>
+__global__ __launch_bounds__(256) void batch_kernel(BatchParams P) {
+    batch_body<Solver, Grp>(P);
 }
 
 // Scratch of the allocation-free entry point (layout in ssw_internal.h): one

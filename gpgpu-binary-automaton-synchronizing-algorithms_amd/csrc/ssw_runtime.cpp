@@ -504,7 +504,10 @@ int ssw_context::launch_pass(const DeviceBatch &db, const ClassSpec &cls, const 
             off += 65536;
             P.lvl_off = off;
             off = align_up(off + 4ull * (65536 + 4), 256);
+            // the choice bitmap of the witness passes, as wide as the LDS
+            // bitmap (zeroed with the slot, left clean by every word)
             P.gbitmap_off = off;
+            off = align_up(off + 4ull * bm_words, 256);
         } else {
             P.hist_off = P.gbitmap_off = P.good_off = P.lvl_off = off;
         }

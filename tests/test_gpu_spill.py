@@ -162,6 +162,7 @@ WIT_CONFIGS = [
     (64, 0, 2, 2048, 1, 0, 0, 0),            # no spill: frontiers move to the HBM ring
     (64, 128, 16, 512, 1, 4096, 4, 0, 15),   # relabelled states, three launches
     (64, 128, 16, 512, 1, 4096, 4, 0, 13),
+    (64, 0, 4, 256, 1, 512, 4, 0),           # small last-launch ring: C_16's good lists outgrow it
 ]
 
 
@@ -175,7 +176,8 @@ def words(case):
 def test_witness_words_every_configuration(ctx, case, words, cfg):
     """Witness words from the n <= 16 path (recorded levels, good marks in the
     LDS bitmap) through every route a search can take: first launch, spill
-    chain, HBM ring."""
+    chain, HBM ring, good-subset lists in LDS and, where they outgrow the
+    ring area, the choice bitmap."""
     A, B, L, S, E = case
     saved = ctx.get_tuning(N)
     try:
