@@ -848,7 +848,7 @@ struct SmallSolver {
     static constexpr uint32_t kDummies = G == 64 ? 64u : (G >= 32 ? 32u : 16u);
     uint32_t dum_a;
     glb_u16 *gring;
-    glb_u16 *hist;   // WIT: subsets in discovery order, 65536 entries
+    glb_u16 *hist;   // WIT: subsets in discovery order, 65536 entries and 64 dummies
     glb_u8 *goodp;   // WIT: 1 = the subset at this position of hist is on a shortest path
     glb_u32 *lvlp;   // WIT: start position of each level in hist
     glb_u32 *choicep;  // WIT: bit x = letter a keeps good subset x on a shortest path
@@ -1414,10 +1414,12 @@ struct SmallSolver {
         uint32_t pos[2 * TT], total;
         g.template place<2 * TT>(sm, m, wr, any_single, pos, total);
         if constexpr (WIT) {
+            // lanes without a new subset write a dummy entry past the 65536
+            // (one per lane) instead of branching around the store
 #pragma unroll
-            for (int j = 0; j < 2 * TT; ++j) {
-                if (nw[j]) hist[pos[j] & 0xFFFFu] = (uint16_t)((j & 1) ? (img[j >> 1] >> 16) : (img[j >> 1] & 0xFFFFu));
-            }
+            for (int j = 0; j < 2 * TT; ++j)
+                hist[nw[j] ? (pos[j] & 0xFFFFu) : 65536u + (r & 63u)] =
+                    (uint16_t)((j & 1) ? (img[j >> 1] >> 16) : (img[j >> 1] & 0xFFFFu));
         }
 #pragma unroll
         for (int j = 0; j < 2 * TT; ++j) {

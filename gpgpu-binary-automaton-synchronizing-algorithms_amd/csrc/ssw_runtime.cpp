@@ -496,10 +496,11 @@ int ssw_context::launch_pass(const DeviceBatch &db, const ClassSpec &cls, const 
         P.garena_cap = 65536;
         off = align_up(2ull * 65536, 256);
         if (wit_small) {
-            // discovery order (u16), a good flag per position (u8) and level
-            // starts (u32) of up to 65536 subsets
+            // discovery order (u16, and 64 dummy entries the search's lanes
+            // without a new subset write), a good flag per position (u8) and
+            // level starts (u32) of up to 65536 subsets
             P.hist_off = off;
-            off += 2ull * 65536;
+            off += 2ull * (65536 + 64);
             P.good_off = off;
             off += 65536;
             P.lvl_off = off;
