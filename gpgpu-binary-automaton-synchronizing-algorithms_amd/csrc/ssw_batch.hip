@@ -458,9 +458,21 @@ __global__ void reset_counters_kernel(uint32_t *deferred, unsigned long long *vi
     for (uint32_t k = threadIdx.x; k < stat_words; k += blockDim.x) launch_stats[k] = 0ull;
 }
 
-template <class Grp>
+// The general solver (any n <= 30, HBM bitmaps past the LDS size, witness
+// words for n > 16): every lane takes T frontier entries per level step, the
+// test-and-sets of all of them in flight together (T > 1: the long searches
+// the small path hands on).
+// FAM (n > 16, no witness words): an automaton with |a(Q)|, |b(Q)| <= 18 is
+// searched in the two-family encoding (every subset past Q lies inside a(Q)
+// or inside b(Q); bit 18 = family B, the subsets of b(Q) not inside a(Q);
+// a(Q) & b(Q) labelled alike in both families) with its visited bitmap of
+// 2^19 bits in LDS, cleared through the arena after each automaton; the
+// others keep the n-bit encoding and the bitmap in HBM.
+template <class Grp, int T = 1, bool FAM = false>
 struct GroupSolver {
     static constexpr uint32_t G = Grp::kSize;
+    static constexpr uint32_t kFamBit = 1u << kFamBits;
+    static constexpr uint32_t kTabBytes = FAM ? (uint32_t)kFamTableBytes : (uint32_t)kTableBytes;
     const BatchParams &P;
     Grp g;
     // LDS
@@ -478,6 +490,10 @@ struct GroupSolver {
     uint32_t Q;
     int32_t nsl;
     bool lds_bm;
+    bool fam = false;         // FAM: this automaton in the two-family encoding
+    bool bm_clean = false;    // FAM: the LDS bitmap is all zero
+    uint32_t fam_nI = 0;      // FAM: the labels of b(Q) outside a(Q)
+    uint32_t e0 = 0;          // subsets counted outside the arena (Q when encoded)
     uint32_t lo, hi, wr, rd, level;
     // work queue
     int64_t it, it_end;
@@ -497,7 +513,7 @@ struct GroupSolver {
     __device__ GroupSolver(const BatchParams &p, void *scratch, uint8_t *gsm, uint8_t *slot)
         : P(p), g(scratch) {
         tab = (lds_u64 *)gsm;
-        bm = (lds_u32 *)(gsm + kTableBytes);
+        bm = (lds_u32 *)(gsm + kTabBytes);
         arena = bm + P.lds_bitmap_words;
         garena = (glb_u32 *)(slot + P.garena_off);
         gbm = (glb_u32 *)(slot + P.gbitmap_off);
@@ -523,11 +539,27 @@ struct GroupSolver {
             garena[p - P.lds_arena_cap] = v;
     }
     __device__ __forceinline__ uint64_t image(uint32_t s) const {
+        if constexpr (FAM) {
+            if (fam) {
+                // family B's slices 0..3 after A's, then the shared slice
+                const uint32_t fb = ((s >> kFamBits) & 1u) << 6;
+                uint64_t r = tab[fb | (s & 15u)] | tab[fb | 16u | ((s >> 4) & 15u)] |
+                             tab[fb | 32u | ((s >> 8) & 15u)] | tab[fb | 48u | ((s >> 12) & 15u)] |
+                             tab[128u | ((s >> 16) & 15u)];
+                // a b-image inside a(Q) belongs to family A
+                if (((uint32_t)(r >> 32) & fam_nI) == 0u) r &= ~((uint64_t)kFamBit << 32);
+                return r;
+            }
+        }
         uint64_t r = 0;
 #pragma unroll
         for (int k = 0; k < 8; ++k)
             if (k < nsl) r |= tab[(k << 4) | ((s >> (4 * k)) & 15u)];
         return r;
+    }
+    // singleton test of an image (without the family bit)
+    __device__ __forceinline__ bool single(uint32_t x) const {
+        return is_single(FAM && fam ? (x & (kFamBit - 1u)) : x);
     }
     // test-and-set: true when this lane set the bit first
     __device__ __forceinline__ bool tas(uint32_t x) const {
@@ -555,11 +587,14 @@ struct GroupSolver {
         nx_ai = item < 0 ? -1 : (P.order ? (int64_t)P.order[item] : item);
     }
     __device__ bool has_next() const { return nx_ai >= 0; }
-    __device__ int begin() { return start() ? (int)kContinue : -1; }
-    __device__ bool start() {
+    __device__ int begin() {
         const int64_t a = nx_ai;
         prefetch();
-        return init(a);
+        if (!init(a)) return -1;
+        if constexpr (FAM) {
+            if (fam) return init_fam();
+        }
+        return kContinue;
     }
 
     // Loads automaton `a`; false when its transitions are invalid.
@@ -567,6 +602,8 @@ struct GroupSolver {
         ai = a;
         n = P.uniform_n > 0 ? P.uniform_n : (int32_t)P.n_per[ai];
         const uint32_t r = g.rank();
+        fam = false;
+        e0 = 0;
         bool bad = n < 1 || n > 30;
         // the group's HBM bitmap must hold 2^n bits when n > lds_n
         if (!bad && !lds_ok_for(n) && (uint64_t)(n >= 5 ? (1u << (n - 5)) : 1u) > P.gbitmap_words)
@@ -589,6 +626,25 @@ struct GroupSolver {
         }
         Q = (1u << n) - 1u;
         lds_bm = lds_ok_for(n);
+        if constexpr (FAM) {
+            // |a(Q)| and |b(Q)| from the transitions (the arena's first words
+            // as accumulators; the tables are built over them below)
+            if (r == 0) {
+                arena[0] = 0u;
+                arena[1] = 0u;
+            }
+            g.sync();
+            for (int q = (int)r; q < n; q += (int)G) {
+                or_wg(arena, 1u << d[q]);
+                or_wg(arena + 1, 1u << d[n + q]);
+            }
+            g.sync();
+            const uint32_t ia = arena[0], ib = arena[1];
+            g.sync();
+            fam = __popc(ia) <= kFamBits && __popc(ib) <= kFamBits;
+            lds_bm = fam;
+            if (fam) return setup_fam(d, ia, ib);
+        }
         if (lds_bm) {
             const uint32_t words = n >= 5 ? (1u << (n - 5)) : 1u;
             if (words >= 4) {
@@ -631,7 +687,88 @@ struct GroupSolver {
         return true;
     }
 
-    __device__ bool lds_ok_for(int32_t nn) const { return nn <= P.lds_n; }
+    __device__ bool lds_ok_for(int32_t nn) const { return !FAM && nn <= P.lds_n; }
+
+    // FAM: the tables of the two-family encoding and the image of Q under
+    // a and b (ia, ib, n-bit masks).  The labels: a(Q) & b(Q) first, in both
+    // families, then the rest of a(Q) (family A) or of b(Q) (family B).
+    uint32_t root_ia = 0, root_ib = 0, root_a = 0, root_b = 0;
+    __device__ bool setup_fam(const uint8_t *d, uint32_t ia, uint32_t ib) {
+        const uint32_t r = g.rank();
+        const uint32_t I = ia & ib;
+        const uint32_t ka = (uint32_t)__popc(ia), kb = (uint32_t)__popc(ib), ki = (uint32_t)__popc(I);
+        const auto lab = [I, ki](uint32_t fs, uint32_t x) {
+            const uint32_t below = (1u << x) - 1u;
+            return ((I >> x) & 1u) ? (uint32_t)__popc(I & below) : ki + (uint32_t)__popc(fs & ~I & below);
+        };
+        // singleton images per family and label (arena words 2.., two u32
+        // halves each), zero past the family's states
+        lds_u64 *pimg = (lds_u64 *)(arena + 2);
+        for (uint32_t k = r; k < 2u * kFamBits; k += G) pimg[k] = 0ull;
+        if (!bm_clean) {
+            for (uint32_t w = r; w < kFamBitmapWords / 4u; w += G) ((lds_u4 *)bm)[w] = u32x4{0u, 0u, 0u, 0u};
+            bm_clean = true;
+        }
+        g.sync();
+        for (int q = (int)r; q < n; q += (int)G) {
+            const uint32_t x = (uint32_t)q;
+            const uint64_t e = (uint64_t)(1u << lab(ia, d[q])) | ((uint64_t)(kFamBit | (1u << lab(ib, d[n + q]))) << 32);
+            if ((ia >> x) & 1u) pimg[lab(ia, x)] = e;
+            if ((ib >> x) & 1u) pimg[kFamBits + lab(ib, x)] = e;
+        }
+        g.sync();
+        // slices 0..3 of family f at 64 f + 16 k + v; the shared slice at 128 + v
+        // (v: bits 0, 1 = labels 16, 17, bit 2 = family)
+        for (uint32_t e = r; e < 144u; e += G) {
+            uint64_t acc = 0;
+            if (e < 128u) {
+                const uint32_t f = e >> 6, k = (e >> 4) & 3u, v = e & 15u;
+#pragma unroll
+                for (uint32_t t = 0; t < 4; ++t)
+                    if ((v >> t) & 1u) acc |= pimg[f * kFamBits + 4u * k + t];
+            } else {
+                const uint32_t v = e - 128u, f = (v >> 2) & 1u;
+                if (!(v & 8u)) {
+                    if (v & 1u) acc |= pimg[f * kFamBits + 16u];
+                    if (v & 2u) acc |= pimg[f * kFamBits + 17u];
+                }
+            }
+            tab[e] = acc;
+        }
+        fam_nI = (kFamBit - 1u) & ~((1u << ki) - 1u);
+        root_ia = ia;
+        root_ib = ib;
+        root_a = (1u << ka) - 1u;
+        const uint32_t rb = (1u << kb) - 1u;
+        root_b = (rb & fam_nI) == 0u ? rb : (rb | kFamBit);
+        g.sync();
+        return true;
+    }
+    // FAM: level 0 (Q) expanded here, Q counted once in E; level 1 = the
+    // new subsets among a(Q), b(Q) in the arena
+    __device__ int init_fam() {
+        const bool na = root_ia != Q, nb = root_ib != Q && root_ib != root_ia;
+        Q = ~0u;  // no encoded subset: the n-bit Q is never met again
+        e0 = 1;
+        if ((na && single(root_a)) || (nb && single(root_b))) {
+            lo = hi = wr = rd = 0;
+            level = 0;
+            found_end = 0;
+            return kFound;
+        }
+        if (g.rank() == 0) {
+            if (na) or_wg(bm + (root_a >> 5), 1u << (root_a & 31u));
+            if (nb) or_wg(bm + (root_b >> 5), 1u << (root_b & 31u));
+            arena[0] = na ? root_a : root_b;
+            arena[1] = root_b;
+        }
+        g.sync();
+        lo = 0;
+        rd = 0;
+        hi = wr = (na ? 1u : 0u) + (nb ? 1u : 0u);
+        level = 1;
+        return hi == 0u ? (int)kNoSync : (int)kContinue;
+    }
 
     __device__ void write_invalid() const {
         P.length[ai] = -3;
@@ -645,33 +782,67 @@ struct GroupSolver {
         for (int32_t k = (int32_t)g.rank(); k < P.witness_stride; k += (int32_t)G) out[k] = 0;
     }
 
-    // One level step over up to G frontier entries.
+    // One level step over up to G * T frontier entries.
     __device__ int step() {
         const uint64_t cap = (uint64_t)P.lds_arena_cap + P.garena_cap;
-        if ((uint64_t)wr + 2u * G > cap) return kOverflow;
+        if ((uint64_t)wr + 2u * G * T > cap) return kOverflow;
         if (P.want_witness && level + 2 >= P.lvl_cap) return kOverflow;
-        const uint32_t p = rd + g.rank();
-        const bool valid = p < hi;
-        const uint32_t s = valid ? aget(p) : 0u;
-        const uint64_t im = image(s);
-        const uint32_t a = (uint32_t)im, b = (uint32_t)(im >> 32);
-        // a singleton equal to Q (n = 1) is already visited: never a find,
-        // as in SlimCPUUntweaked.cs:149-151
-        const bool sa = valid && is_single(a) && a != Q;
-        const bool sb = valid && is_single(b) && b != Q;
-        const bool na = valid && !sa && tas(a);
-        const bool nb = valid && !sb && tas(b);
         bool any_single;
-        uint32_t off, total;
-        g.exchange(sa || sb, na, nb, any_single, off, total);
-        if (na) aput(wr + off, a);
-        if (nb) aput(wr + off + (na ? 1u : 0u), b);
+        uint32_t total;
+        if constexpr (T == 1) {
+            const uint32_t p = rd + g.rank();
+            const bool valid = p < hi;
+            const uint32_t s = valid ? aget(p) : 0u;
+            const uint64_t im = image(s);
+            const uint32_t a = (uint32_t)im, b = (uint32_t)(im >> 32);
+            // a singleton equal to Q (n = 1) is already visited: never a find,
+            // as in SlimCPUUntweaked.cs:149-151
+            const bool sa = valid && single(a) && a != Q;
+            const bool sb = valid && single(b) && b != Q;
+            const bool na = valid && !sa && tas(a);
+            const bool nb = valid && !sb && tas(b);
+            uint32_t off;
+            g.exchange(sa || sb, na, nb, any_single, off, total);
+            if (na) aput(wr + off, a);
+            if (nb) aput(wr + off + (na ? 1u : 0u), b);
+        } else {
+            // entries rd + t G + rank; the new subsets of a lane are placed
+            // together (lane-major), which leaves lengths and expanded counts
+            // as they are (a level's order does not enter them)
+            uint32_t v[2 * T];
+            bool f[2 * T];
+            bool found = false;
+#pragma unroll
+            for (int t = 0; t < T; ++t) {
+                const uint32_t p = rd + (uint32_t)t * G + g.rank();
+                const bool valid = p < hi;
+                const uint32_t s = valid ? aget(p) : 0u;
+                const uint64_t im = image(s);
+                v[2 * t] = (uint32_t)im;
+                v[2 * t + 1] = (uint32_t)(im >> 32);
+                const bool sa = valid && single(v[2 * t]) && v[2 * t] != Q;
+                const bool sb = valid && single(v[2 * t + 1]) && v[2 * t + 1] != Q;
+                found |= sa || sb;
+                f[2 * t] = valid && !sa;
+                f[2 * t + 1] = valid && !sb;
+            }
+#pragma unroll
+            for (int j = 0; j < 2 * T; ++j) f[j] = f[j] && tas(v[j]);
+            uint32_t pos;
+            g.template exchange_flags<2 * T>(found, f, any_single, pos, total);
+            pos += wr;
+#pragma unroll
+            for (int j = 0; j < 2 * T; ++j) {
+                if (f[j]) aput(pos, v[j]);
+                pos += f[j] ? 1u : 0u;
+            }
+        }
         wr += total;
         if (any_single) {
-            found_end = rd + G < hi ? rd + G : hi;
+            found_end = rd + G * T < hi ? rd + G * T : hi;
             return kFound;
         }
-        rd += G;
+        rd += G * T;
         if (rd >= hi) {
             g.sync();
             ++level;
@@ -763,7 +934,7 @@ struct GroupSolver {
         } else {
             const bool found = code == kFound;
             const int32_t L = found ? (int32_t)level + 1 : -1;
-            const int64_t E = hi;
+            const int64_t E = (int64_t)hi + e0;
             if (r == 0) {
                 P.length[ai] = L;
                 if (P.sync) P.sync[ai] = found ? 1 : 0;
@@ -775,11 +946,14 @@ struct GroupSolver {
             if (!found && P.witness) clear_witness_row();
             ++done_local;
             subsets_local += (uint64_t)E;
-            expanded_local += found ? found_end : hi;
+            expanded_local += (uint64_t)(found ? found_end : hi) + e0;
         }
         if (!lds_bm) {
             g.sync();
             for (uint32_t p = r; p < wr; p += G) gbm[aget(p) >> 5] = 0;
+        } else if (FAM && fam) {
+            g.sync();
+            for (uint32_t p = r; p < wr; p += G) bm[aget(p) >> 5] = 0;
         }
         g.sync();
         if (done_local >= 256) flush();
@@ -828,9 +1002,20 @@ struct GroupSolver {
 // the levels (good marks of the next level in the LDS bitmap, which the
 // search no longer needs, and one flag per recorded position in HBM) and a
 // forward greedy pass.
-template <class Grp, int T, bool BT, bool WIT = false>
+// FAM (n > 16, one wavefront, T = 1, nibble tables): every subset reached
+// from Q but Q itself lies inside a(Q) (the images under a) or inside b(Q)
+// (those under b), so it is encoded in 16 bits as a family and a mask of at
+// most 15 relabelled states: family A (bit 15 clear) holds the subsets of
+// a(Q), family B (bit 15 set) those of b(Q) that are not inside a(Q).  The
+// states of a(Q) & b(Q) take the lowest labels in both families, so a
+// b-image inside a(Q) becomes family A by clearing bit 15.  The encoding is
+// one to one on the reachable subsets, so lengths and expanded counts are
+// those of the search over n-bit masks; automata with |a(Q)| or |b(Q)| above
+// 15 are handed on.
+template <class Grp, int T, bool BT, bool WIT = false, bool FAM = false>
 struct SmallSolver {
     static_assert(!WIT || Grp::kSize == 64, "witness words: one wavefront per automaton");
+    static_assert(!FAM || (Grp::kSize == 64 && T == 1 && !BT && !WIT), "two-family encoding: one wavefront, T = 1");
     static constexpr uint32_t G = Grp::kSize;
     static constexpr uint32_t kGlobalRing = 65536;
     // one persistent loop per automaton when a group is a whole wavefront or
@@ -856,6 +1041,8 @@ struct SmallSolver {
     bool in_lds;
     bool narrow;  // relabelled states, visited bitmap of 2^narrow_bits bits
     uint32_t root_a, root_b;  // n > 16: a(Q), b(Q) in the relabelled states
+    uint32_t fam_nI;          // FAM: the labels of b(Q) outside a(Q) (family B only)
+    uint32_t fam_ia, fam_ib;  // FAM: a(Q), b(Q) in the n-bit labels
     uint32_t e0;              // subsets counted outside the ring (Q when n > 16)
     int64_t ai;
     int32_t n;
@@ -882,7 +1069,7 @@ struct SmallSolver {
         // a mask in an SGPR and the base (one scalar operand per instruction)
         bm_a = base + gib * bm_bytes;
         tab_a = base + SmallLds::regions_base(groups, bm_bytes) + gib * P.lds_group_bytes;
-        stage_a = tab_a + SmallLds::tab_bytes(BT);
+        stage_a = tab_a + SmallLds::tab_bytes(BT, FAM);
         pimg_a = stage_a + 64;
         ctl_a = pimg_a + 64;
         ring_a = ctl_a + 128;
@@ -926,7 +1113,15 @@ struct SmallSolver {
     // all slices are built (slices past n hold zeros), so the lookups are
     // unconditional and issue back to back
     __device__ __forceinline__ uint32_t image(uint32_t s) const {
-        if constexpr (BT) {
+        if constexpr (FAM) {
+            // slices 0..2 per family (family B's 256 bytes above A's), slice
+            // 3 shared: its index holds the family bit
+            const uint32_t fb = tab_a + ((s >> 7) & 0x100u);
+            const uint32_t im = *l32(fb | ((s << 2) & 60u)) | *l32((fb | ((s >> 2) & 60u)) + 64u) |
+                                *l32((fb | ((s >> 6) & 60u)) + 128u) | *l32((tab_a | ((s >> 10) & 60u)) + 192u);
+            // a b-image inside a(Q) belongs to family A
+            return ((im >> 16) & fam_nI) == 0u ? (im & 0x7FFFFFFFu) : im;
+        } else if constexpr (BT) {
             return *l32(tab_a | ((s << 2) & 0x3FCu)) | *l32((tab_a | ((s >> 6) & 0x3FCu)) + 1024u);
         } else {
             return *l32(tab_a | ((s << 2) & 60u)) | *l32((tab_a | ((s >> 2) & 60u)) + 64u) |
@@ -1003,7 +1198,39 @@ struct SmallSolver {
             qb = *l8(stage_a + (uint32_t)n + r);
         }
         uint32_t words;
-        if (narrow) {
+        if constexpr (FAM) {
+            // the two families (the encoding is described above the solver);
+            // lanes 0..n-1 hold the states (n <= 30, one wavefront)
+            const uint32_t ia = g.or_low32(r < (uint32_t)n ? 1u << qa : 0u);
+            const uint32_t ib = g.or_low32(r < (uint32_t)n ? 1u << qb : 0u);
+            const uint32_t I = ia & ib;
+            const uint32_t ka = (uint32_t)__popc(ia), kb = (uint32_t)__popc(ib), ki = (uint32_t)__popc(I);
+            if (ka > 15u || kb > 15u) return spill_to(P.spill_queue, P.spill_list, 0u);
+            // label of state x in the family of `fam` (a(Q) or b(Q)): a(Q) & b(Q) first
+            const auto lab = [I, ki](uint32_t fam, uint32_t x) {
+                const uint32_t below = (1u << x) - 1u;
+                return ((I >> x) & 1u) ? (uint32_t)__popc(I & below) : ki + (uint32_t)__popc(fam & ~I & below);
+            };
+            // singleton images per family and label (family B's in the ring
+            // area until the tables are built), zero past the family's states
+            if (r < 16u) {
+                *l32(pimg_a + 4u * r) = 0u;
+                *l32(ring_a + 4u * r) = 0u;
+            }
+            lsync();
+            if (r < (uint32_t)n) {
+                const uint32_t e = (1u << lab(ia, qa)) | ((0x8000u | (1u << lab(ib, qb))) << 16);
+                if ((ia >> r) & 1u) *l32(pimg_a + 4u * lab(ia, r)) = e;
+                if ((ib >> r) & 1u) *l32(ring_a + 4u * lab(ib, r)) = e;
+            }
+            fam_nI = 0x7FFFu & ~((1u << ki) - 1u);
+            fam_ia = ia;
+            fam_ib = ib;
+            root_a = (1u << ka) - 1u;
+            const uint32_t rb = (1u << kb) - 1u;
+            root_b = (rb & fam_nI) == 0u ? rb : (rb | 0x8000u);
+            words = 2048u;
+        } else if (narrow) {
             // Every subset reached from Q but Q itself lies inside
             // U = a(Q) | b(Q) (the image of any subset is inside the image of
             // Q).  The states are relabelled so that U takes the |U| lowest
@@ -1051,7 +1278,20 @@ struct SmallSolver {
             for (uint32_t w = r; w < words; w += G) *l32(bm_a + 4u * w) = 0u;
         }
         lsync();
-        if constexpr (BT) {
+        if constexpr (FAM) {
+            // family f's slices 0..2 at tab_a + 256 f + 64 j, entry v = OR of
+            // the images of labels 4j + (bits of v); slice 3 at tab_a + 192,
+            // entry v = family bit (8) and labels 12..14 (bits 0..2)
+            for (uint32_t e = r; e < 112u; e += G) {
+                const uint32_t f = e < 96u ? e / 48u : (e - 96u) >> 3;
+                const uint32_t j = e < 96u ? (e % 48u) >> 4 : 3u;
+                const uint32_t v = e < 96u ? (e & 15u) : (e & 7u);
+                const u32x4 p4 = *l128((f ? ring_a : pimg_a) + 16u * j);
+                const uint32_t val =
+                    ((v & 1u) ? p4.x : 0u) | ((v & 2u) ? p4.y : 0u) | ((v & 4u) ? p4.z : 0u) | ((v & 8u) ? p4.w : 0u);
+                *l32(e < 96u ? tab_a + 256u * f + 64u * j + 4u * v : tab_a + 192u + 4u * (e - 96u)) = val;
+            }
+        } else if constexpr (BT) {
             // 64 blocks of 8 entries: slice k = blk >> 5, entries v = 8 * (blk & 31) + i
             for (uint32_t blk = r; blk < 64u; blk += G) {
                 const uint32_t k = blk >> 5, h5 = blk & 31u;
@@ -1072,7 +1312,7 @@ struct SmallSolver {
             }
         }
         lsync();
-        if constexpr (G == 64 && !WIT) {
+        if constexpr (G == 64 && !WIT && !FAM) {
             // Verify mode: lane r applies the word of length 6 whose letter k
             // is b when bit k of r is set (the 64 words one wavefront holds;
             // SuperSlimGPUBreakthrough.cs:183-256 does the same with one
@@ -1093,7 +1333,10 @@ struct SmallSolver {
         // itself) seeds the ring, and Q counts once in E.  With no new
         // subset Q is the only reachable one: not synchronizing, E = 1 (this
         // covers n = 1, where Q is already a singleton).
-        const bool na = root_a != Q, nb = root_b != Q && root_b != root_a;
+        // (encoded: compared in the n-bit labels, where a permutation letter
+        // maps Q to itself)
+        const bool na = FAM ? fam_ia != Q : root_a != Q;
+        const bool nb = FAM ? (fam_ib != Q && fam_ib != fam_ia) : (root_b != Q && root_b != root_a);
         if (r == 0) {
             // relabelled, Q lies outside the narrow bitmap and is never reached again
             if (!narrow) or_wg(l32(bm_a | ((Q >> 3) & 0x1FFCu)), 1u << (Q & 31u));
@@ -1112,7 +1355,7 @@ struct SmallSolver {
         lsync();
         rd = 0;
         e0 = 1;
-        if ((na && is_single(root_a)) || (nb && is_single(root_b))) {
+        if ((na && is_single(root_a)) || (nb && is_single(FAM ? (root_b & 0x7FFFu) : root_b))) {
             hi = wr = 0;
             level = 0;
             found_end = 0;
@@ -1376,7 +1619,7 @@ struct SmallSolver {
         sm = 0;
 #pragma unroll
         for (int t = 0; t < TT; ++t) {
-            const uint32_t x = img[t];
+            const uint32_t x = FAM ? (img[t] & 0x7FFF7FFFu) : img[t];  // without the family bit
             const uint32_t y = x & (x - 0x00010001u);
             // a half of y is zero: min of the halves (one v_min_u16 with a
             // word select) is zero
@@ -1988,7 +2231,10 @@ static hipError_t occ_t(int block, size_t lds, int *blocks) {
 // and nibble tables; 4..6 = small path with T = 1, 2, 4 and byte tables; 7 =
 // small path with T = 8 and byte tables, one wavefront per automaton; 8, 9 =
 // small path with witness words (one wavefront per automaton), T = 1 with
-// nibble tables and T = 4 with byte tables
+// nibble tables and T = 4 with byte tables; 10 = small path with the
+// two-family encoding of n > 16 (one wavefront, T = 1, nibble tables); 11 =
+// general with four entries per lane (groups of 64 or 256 lanes); 12 = the
+// same with the two-family encoding and the LDS bitmap of 2^19 bits
 #define SSW_SMALL_CASES(FN, V, TT, BT, ...)                                                                   \
     case V * 1000 + 16: return FN<SmallSolver<WaveGroup<16>, TT, BT>, WaveGroup<16>>(__VA_ARGS__);            \
     case V * 1000 + 32: return FN<SmallSolver<WaveGroup<32>, TT, BT>, WaveGroup<32>>(__VA_ARGS__);            \
@@ -2001,6 +2247,10 @@ static hipError_t occ_t(int block, size_t lds, int *blocks) {
         case 32: return FN<GroupSolver<WaveGroup<32>>, WaveGroup<32>>(__VA_ARGS__);         \
         case 64: return FN<GroupSolver<WaveGroup<64>>, WaveGroup<64>>(__VA_ARGS__);         \
         case 256: return FN<GroupSolver<BlockGroup<256>>, BlockGroup<256>>(__VA_ARGS__);    \
+        case 11064: return FN<GroupSolver<WaveGroup<64>, 4>, WaveGroup<64>>(__VA_ARGS__);  \
+        case 11256: return FN<GroupSolver<BlockGroup<256>, 4>, BlockGroup<256>>(__VA_ARGS__); \
+        case 12064: return FN<GroupSolver<WaveGroup<64>, 4, true>, WaveGroup<64>>(__VA_ARGS__); \
+        case 12256: return FN<GroupSolver<BlockGroup<256>, 4, true>, BlockGroup<256>>(__VA_ARGS__); \
         SSW_SMALL_CASES(FN, 1, 1, false, __VA_ARGS__)                                      \
         SSW_SMALL_CASES(FN, 2, 2, false, __VA_ARGS__)                                      \
         SSW_SMALL_CASES(FN, 3, 4, false, __VA_ARGS__)                                      \
@@ -2010,6 +2260,7 @@ static hipError_t occ_t(int block, size_t lds, int *blocks) {
         case 7064: return FN<SmallSolver<WaveGroup<64>, 8, true>, WaveGroup<64>>(__VA_ARGS__); \
         case 8064: return FN<SmallSolver<WaveGroup<64>, 1, false, true>, WaveGroup<64>>(__VA_ARGS__); \
         case 9064: return FN<SmallSolver<WaveGroup<64>, 4, true, true>, WaveGroup<64>>(__VA_ARGS__); \
+        case 10064: return FN<SmallSolver<WaveGroup<64>, 1, false, false, true>, WaveGroup<64>>(__VA_ARGS__); \
         default: return hipErrorInvalidValue;                                              \
     }
 
@@ -2032,14 +2283,17 @@ size_t batch_scratch_bytes(int group_lanes) {
 static bool variant_byte_tab(int variant) { return (variant >= 4 && variant <= 7) || variant == 9; }
 
 size_t batch_group_lds(int variant, uint32_t bitmap_words, uint32_t arena_cap) {
-    return variant == 0 ? (size_t)(kTableBytes + 4ull * bitmap_words + 4ull * arena_cap)
-                        : (size_t)SmallLds::region_bytes(variant_byte_tab(variant), arena_cap);
+    if (variant == 12) return (size_t)(kFamTableBytes + 4ull * bitmap_words + 4ull * arena_cap);
+    return variant == 0 || variant == 11 ? (size_t)(kTableBytes + 4ull * bitmap_words + 4ull * arena_cap)
+                        : (size_t)SmallLds::region_bytes(variant_byte_tab(variant), arena_cap, variant == 10);
 }
 
 size_t batch_block_lds(int variant, int group_lanes, int groups, uint32_t bitmap_words, uint32_t arena_cap) {
     const size_t scratch = batch_scratch_bytes(group_lanes);
-    if (variant == 0) return batch_group_lds(0, bitmap_words, arena_cap) * groups + scratch;
-    return SmallLds::block_bytes((uint32_t)groups, 4u * bitmap_words, SmallLds::region_bytes(variant_byte_tab(variant), arena_cap),
+    if (variant == 0 || variant == 11 || variant == 12)
+        return batch_group_lds(variant, bitmap_words, arena_cap) * groups + scratch;
+    return SmallLds::block_bytes((uint32_t)groups, 4u * bitmap_words,
+                                 SmallLds::region_bytes(variant_byte_tab(variant), arena_cap, variant == 10),
                                  (uint32_t)scratch);
 }
 

@@ -81,6 +81,9 @@ struct PassIO {
     int narrow_bits = 0;                // relabelled states, 2^narrow_bits-bit bitmap
     bool byte_tab = false;
     bool force_small = false;           // the n <= 16 solver on a class of larger n (relabelled)
+    bool fam = false;                   // ... with the two-family encoding (n > 16)
+    int general_entries = 1;            // frontier entries per lane of the general solver (1 or 4)
+    bool general_fam = false;           // ... with the two-family encoding (n > 16, 4 entries per lane)
     bool queue_ready = false;           // the class's queue was set up with the spill queues
 };
 constexpr int kClassQueues = 3;  // work queues per launch class: its own and two spill queues

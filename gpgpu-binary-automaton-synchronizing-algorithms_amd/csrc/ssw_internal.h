@@ -12,6 +12,12 @@ namespace ssw {
 constexpr int kSliceBits = 4;
 constexpr int kTableEntries = 8 * 16;
 constexpr int kTableBytes = kTableEntries * 8;
+// The general solver's two-family encoding (n > 16, visited bitmap in LDS):
+// a family bit and a mask of at most kFamBits relabelled states; its tables
+// hold slices 0..3 per family and one shared slice for bits 16..19.
+constexpr int kFamBits = 18;
+constexpr int kFamTableBytes = (2 * 4 * 16 + 16) * 8;
+constexpr uint32_t kFamBitmapWords = (1u << (kFamBits + 1)) / 32;
 
 // Device work queue: kQueues segments of kQueueWords u32 each (next, end),
 // then a u64 mask of exhausted segments.
@@ -50,10 +56,13 @@ struct SmallLds {
         const uint32_t b = n >= 3 ? (1u << (n - 3)) : 1u;
         return b < 256u ? 256u : b;
     }
-    static __host__ __device__ inline uint32_t tab_bytes(bool byte_tab) { return byte_tab ? 2048u : 256u; }
-    static __host__ __device__ inline uint32_t region_bytes(bool byte_tab, uint32_t ring_cap) {
+    // byte tables; nibble tables (two families: slices 0..2 twice)
+    static __host__ __device__ inline uint32_t tab_bytes(bool byte_tab, bool fam = false) {
+        return byte_tab ? 2048u : (fam ? 512u : 256u);
+    }
+    static __host__ __device__ inline uint32_t region_bytes(bool byte_tab, uint32_t ring_cap, bool fam = false) {
         // tables, staging, singleton images, control words, spare, ring (+64 dummy slots)
-        const uint32_t b = tab_bytes(byte_tab) + 64u + 64u + 64u + 64u + 2u * (ring_cap + 64u);
+        const uint32_t b = tab_bytes(byte_tab, fam) + 64u + 64u + 64u + 64u + 2u * (ring_cap + 64u);
         const uint32_t a = byte_tab ? 1024u : 256u;
         return (b + a - 1u) / a * a;
     }

@@ -245,27 +245,39 @@ int ssw_context::launch_class(const DeviceBatch &db, const ClassSpec &cls, bool 
     }
     if (!plain && !small && db.witness == nullptr && cls.nmax <= SSW_MAX_N && t.narrow_bits == 16 &&
         cls.spill_queue[0]) {
-        // n > 16: automata with |a(Q) | b(Q)| <= 16 run relabelled in the
-        // n <= 16 solver (LDS bitmap, 16-bit ring; the first launch
-        // configuration of n = 16); the others, and frontiers that outgrow its
-        // ring, go on to this class's solver from a spill queue
+        // n > 16: automata with |a(Q)|, |b(Q)| <= 15 run in the n <= 16
+        // solver with the two-family encoding (LDS bitmap, 16-bit ring; the
+        // first launch configuration of n = 16); the others, and frontiers
+        // that outgrow its ring, go on to this class's solver from a spill
+        // queue
         hipStream_t st = cls.stream ? cls.stream : stream;
         HIP_TRY(init_queues(cls.spill_queue[0], cls.items, st, true), "init queues");
         ssw_tuning tp = tune[kMaxLdsN];
         tp.group_lanes = 64;
         tp.lane_entries = 1;
+        // a ring of 2048 entries: the frontiers of n > 16 are wider than
+        // those of n = 16 and a search that outgrows the ring starts over in
+        // the slower solver
+        tp.arena_lds = std::max(tp.arena_lds, 2048);
+        tp.block_threads = 0;
         PassIO pre;
         pre.kind = kPassSpillFrom;
         pre.out_list = cls.spill_list[0];
         pre.out_queue = cls.spill_queue[0];
         pre.narrow_bits = 16;
         pre.force_small = true;
+        pre.fam = true;
         int rc = launch_pass(db, cls, tp, false, pre);
         if (rc) return rc;
         PassIO rest;
         rest.kind = kPassSpillTo;
         rest.in_list = cls.spill_list[0];
         rest.in_queue = cls.spill_queue[0];
+        // what is left are the long searches (a letter close to a
+        // permutation): four frontier entries per lane and step, and with
+        // |a(Q)|, |b(Q)| <= 18 the two-family encoding with its bitmap in LDS
+        rest.general_entries = 4;
+        rest.general_fam = true;
         return launch_pass(db, cls, t, false, rest);
     }
     if (plain || !small || t.spill_arena_lds <= 0 || !cls.spill_queue[0])
@@ -405,7 +417,9 @@ int ssw_context::launch_pass(const DeviceBatch &db, const ClassSpec &cls, const 
     const int T = wit_small ? (byte_tab ? 4 : 1)
                   : (t.lane_entries >= 8 && byte_tab && G == 64) ? 8
                   : (t.lane_entries >= 4 ? 4 : (t.lane_entries >= 2 ? 2 : 1));
-    const int variant = !small ? 0
+    const bool general4 = !small && io.general_entries >= 4 && !plain && (G == 64 || G == 256);
+    const int variant = !small ? (general4 ? (io.general_fam && !want_wit ? 12 : 11) : 0)
+                        : io.fam ? 10
                         : wit_small ? (byte_tab ? 9 : 8)
                         : (T == 8 ? 7 : (T == 1 ? 1 : (T == 2 ? 2 : 3)) + (byte_tab ? 3 : 0));
 
@@ -442,7 +456,8 @@ int ssw_context::launch_pass(const DeviceBatch &db, const ClassSpec &cls, const 
     P.chunk = (uint32_t)std::max(1, t.chunk);
     P.lds_n = use_lds ? nmax : 0;
     const uint32_t bm_words = small ? SmallLds::bitmap_bytes(P.narrow_bits > 0 ? P.narrow_bits : nmax) / 4u
-                                    : (use_lds ? (uint32_t)align_up(nmax >= 5 ? (1u << (nmax - 5)) : 1u, 4) : 4u);
+                              : variant == 12 ? kFamBitmapWords
+                              : (use_lds ? (uint32_t)align_up(nmax >= 5 ? (1u << (nmax - 5)) : 1u, 4) : 4u);
     P.lds_bitmap_words = bm_words;
     const uint64_t all_subsets = 1ull << nmax;
     uint32_t arena_lds;
@@ -456,6 +471,9 @@ int ssw_context::launch_pass(const DeviceBatch &db, const ClassSpec &cls, const 
     } else {
         arena_lds = (uint32_t)align_up((uint64_t)std::max(4, t.arena_lds), 4);
         if (arena_lds > all_subsets + 2u * G) arena_lds = (uint32_t)align_up(all_subsets + 2u * G, 4);
+        // with the 64 KiB bitmap: two 256-lane groups per CU (the arena
+        // continues in HBM; its first words hold the encoding's set-up)
+        if (variant == 12) arena_lds = std::max(256u, std::min(arena_lds, 2048u));
     }
     P.lds_arena_cap = arena_lds;
     const size_t per_group = batch_group_lds(variant, bm_words, arena_lds);
@@ -591,8 +609,8 @@ int ssw_context::launch_pass(const DeviceBatch &db, const ClassSpec &cls, const 
             r.grid = grid;
             r.lds = (int)lds;
             r.ring = (int)arena_lds;
-            r.T = small ? T : 1;
-            r.bitmap_bits = use_lds ? (int)(32u * bm_words) : 0;
+            r.T = small ? T : (variant >= 11 ? 4 : 1);
+            r.bitmap_bits = (use_lds || variant == 12) ? (int)(32u * bm_words) : 0;
             r.pass = io.kind == kPassAll && !io.out_list ? 0 : (io.in_list ? 2 : 1);
             r.byte_tab = byte_tab;
             r.witness = want_wit;
@@ -672,11 +690,21 @@ int ssw_context::solve_device(const DeviceBatch &db, const uint8_t *host_n, bool
         // All classes run concurrently on their own streams, each with its
         // own device work queue; inside a class the work order puts the
         // largest n first so the long searches start early.
+        // A batch of mixed sizes above 16 without witness words is one class
+        // instead: the two-family launch takes every size at once (its
+        // tables are built per automaton) and the general solver what it
+        // hands on, so the sizes do not queue behind each other in launches
+        // that each fill the GPU.
         std::map<std::tuple<int, int, int>, std::vector<int>> classes;  // (G or 0, lds, n or 0) -> sizes
+        const bool merged = db.uniform_n <= 0 && db.witness == nullptr && nmaxall > kMaxLdsN &&
+                            tune[nmaxall].narrow_bits == 16;
         for (int n = 1; n <= SSW_MAX_N; ++n) {
             if (!hist[n]) continue;
             const bool lds = n <= kMaxLdsN;
-            classes[std::make_tuple(lds ? 0 : tune[n].group_lanes, lds ? 1 : 0, lds ? n : 0)].push_back(n);
+            if (merged)
+                classes[std::make_tuple(0, 0, 0)].push_back(n);
+            else
+                classes[std::make_tuple(lds ? 0 : tune[n].group_lanes, lds ? 1 : 0, lds ? n : 0)].push_back(n);
         }
         const bool single_class = classes.size() == 1 && db.uniform_n > 0;
         std::vector<int64_t> class_start;
@@ -1218,11 +1246,13 @@ int ssw_context::launch_stats(ssw_launch_stat *out, int32_t cap, int32_t *count)
         o.subsets = c[3 * k + 1];
         o.expanded = c[3 * k + 2];
         const bool wave = r.G <= 64;
-        if (r.variant == 0)
-            std::snprintf(o.kernel, sizeof o.kernel, "GroupSolver<%s<%d>>", wave ? "WaveGroup" : "BlockGroup", r.G);
+        if (r.variant == 0 || r.variant == 11 || r.variant == 12)
+            std::snprintf(o.kernel, sizeof o.kernel, "GroupSolver<%s<%d>%s>", wave ? "WaveGroup" : "BlockGroup", r.G,
+                          r.variant == 11 ? ",4" : (r.variant == 12 ? ",4,true" : ""));
         else
             std::snprintf(o.kernel, sizeof o.kernel, "SmallSolver<%s<%d>,%d,%s%s>", wave ? "WaveGroup" : "BlockGroup",
-                          r.G, r.T, r.byte_tab ? "true" : "false", r.witness ? ",true" : "");
+                          r.G, r.T, r.byte_tab ? "true" : "false",
+                          r.variant == 10 ? ",false,true" : (r.witness ? ",true" : ""));
     }
     return SSW_OK;
 }

@@ -156,6 +156,86 @@ def test_relabelled_large_n(ctx):
         np.testing.assert_array_equal(r.expanded[idx], E, err_msg=f"n={n}")
 
 
+def _family_cases(rng):
+    """Automata of n = 17..24 that probe the two-family encodings: a(Q) and
+    b(Q) overlapping in few or many states, b(Q) inside a(Q), a constant
+    letter whose state lies outside a(Q) (a family-B singleton at the root),
+    |a(Q)| = 15 (the n <= 16 solver's 15-state families), 16..18 (handed on to
+    the general solver's 18-state families), 19 and Cerny C_n for n > 18 (its
+    HBM bitmap); Cerny C_17, C_18 have a permutation letter inside 18 states."""
+    cases = []
+    for n in range(17, 25):
+        for k in range(40):
+            kind = k % 8
+            if kind == 0:     # a into 0..14, b into n-15..n-1: small overlap
+                a = rng.integers(0, 15, size=n)
+                b = rng.integers(n - 15, n, size=n)
+            elif kind == 1:   # large overlap
+                a = rng.integers(0, 15, size=n)
+                b = rng.integers(2, 17, size=n)
+            elif kind == 2:   # b(Q) inside a(Q)
+                a = rng.integers(0, 15, size=n)
+                b = rng.integers(0, 15, size=n) % 8
+            elif kind == 3:   # b constant outside a(Q): length 1
+                a = rng.integers(0, 10, size=n)
+                b = np.full(n, n - 1)
+            elif kind == 4:   # exactly 15 states in a(Q)
+                a = np.concatenate([np.arange(15), rng.integers(0, 15, size=n - 15)])
+                b = rng.integers(0, n, size=n)
+            elif kind == 5:   # 16..19 states in a(Q): handed on (up to 18 encoded there)
+                m = min(n, 16 + (k // 8) % 4)
+                a = np.concatenate([np.arange(m), rng.integers(0, m, size=n - m)])
+                b = rng.integers(0, n, size=n)
+            elif kind == 6:
+                a, b = ssw.generate_cerny(n, k % 2)
+            else:             # plain random
+                a = rng.integers(0, n, size=n)
+                b = rng.integers(0, n, size=n)
+            perm = rng.permutation(n)  # relabel the states: any label may hold any role
+            inv = np.argsort(perm)
+            a = perm[np.asarray(a)[inv]]
+            b = perm[np.asarray(b)[inv]]
+            cases.append((n, np.asarray(a, np.int32), np.asarray(b, np.int32)))
+    return cases
+
+
+def test_two_family_encoding(ctx):
+    """n = 17..24 in the two-family encodings (subsets of a(Q) or of b(Q),
+    a(Q) & b(Q) labelled alike in both): the n <= 16 solver's (15 states per
+    family) and, for what it hands on, the general solver's (18 states, LDS
+    bitmap; HBM bitmap past that).  Lengths, flags and expanded counts against
+    the oracle's uint32 search, and most of the probe automata finished by
+    the first encoded launch."""
+    rng = np.random.default_rng(57)
+    cases = _family_cases(rng)
+    ns = np.array([c[0] for c in cases], np.uint8)
+    As = [c[1] for c in cases]
+    Bs = [c[2] for c in cases]
+    saved = {n: ctx.get_tuning(n) for n in range(17, 25)}
+    try:
+        for n in range(17, 25):
+            t = dict(saved[n])
+            t["narrow_bits"] = 16
+            ctx.set_tuning(n, **t)
+        r = ctx.solve_mixed(ns, As, Bs)
+        stats = ctx.last_launch_stats()
+    finally:
+        for n, t in saved.items():
+            ctx.set_tuning(n, **t)
+    for n in range(17, 25):
+        idx = np.nonzero(ns == n)[0]
+        A = np.stack([As[i] for i in idx])
+        B = np.stack([Bs[i] for i in idx])
+        L, S, E, _ = O.solve(n, A, B, threads=8)
+        np.testing.assert_array_equal(r.length[idx], L, err_msg=f"n={n}")
+        np.testing.assert_array_equal(r.sync[idx], S, err_msg=f"n={n}")
+        np.testing.assert_array_equal(r.expanded[idx], E, err_msg=f"n={n}")
+    fam = [x for x in stats if x["kernel"].endswith(",false,true>")]
+    assert fam, [x["kernel"] for x in stats]
+    # kinds 0..4 and 7 (6 of every 8) fit the encoding; a few may outgrow the ring
+    assert sum(x["automata"] for x in fam) >= 0.6 * len(cases), [(x["n"], x["automata"]) for x in fam]
+
+
 WIT_CONFIGS = [
     (64, 128, 16, 512, 1, 4096, 4, 0),       # the committed spill chain
     (64, 0, 4, 256, 1, 2048, 4, 0),          # last launch's ring too small for C_16: HBM ring
