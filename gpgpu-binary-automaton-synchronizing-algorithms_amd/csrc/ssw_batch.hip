@@ -2342,6 +2342,28 @@ hipError_t noalloc_scratch(int device, void **out) {
 // library serialises them per GPU with this lock (within one process).
 std::mutex &noalloc_call_lock(int device) { return g_noalloc_call_mu[device < 0 || device >= kMaxDevices ? 0 : device]; }
 
+// Counts of the sizes n_per[0..count) into hist[0..31] (31: n >= 31), which
+// the caller has cleared.
+__global__ void count_sizes_kernel(const uint8_t *n_per, int64_t count, unsigned long long *hist) {
+    __shared__ uint32_t h[32];
+    if (threadIdx.x < 32) h[threadIdx.x] = 0u;
+    __syncthreads();
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t n = n_per[i];
+        atomicAdd(&h[n < 31u ? n : 31u], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < 32 && h[threadIdx.x]) atomicAdd(hist + threadIdx.x, (unsigned long long)h[threadIdx.x]);
+}
+
+hipError_t count_sizes(const uint8_t *n_per, int64_t count, unsigned long long *hist, hipStream_t st) {
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, st, (uint32_t *)hist, (uint64_t)64, 0u);
+    const int64_t want = (count + 1023) / 1024;
+    const int64_t blocks = want < 1 ? 1 : (want > 1024 ? 1024 : want);
+    hipLaunchKernelGGL(count_sizes_kernel, dim3((unsigned)blocks), dim3(256), 0, st, n_per, count, hist);
+    return hipGetLastError();
+}
+
 hipError_t fill_u32(uint32_t *p, uint64_t words, uint32_t value, hipStream_t st) {
     if (words == 0) return hipSuccess;
     uint64_t blocks = (words + 255) / 256;

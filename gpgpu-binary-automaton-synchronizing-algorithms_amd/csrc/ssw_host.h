@@ -186,6 +186,7 @@ struct ssw_context {
     std::string err;
 
     ssw::DevBuf ctrs, queues, deferred, order, spill;
+    ssw::DevBuf sizes;  // counts of a device batch's sizes (32 u64)
     // per-launch statistics of the last batched solve: kMaxLaunchRecords
     // triples of u64 counters (zeroed with the solve's counters), the events
     // around each launch and its shape
@@ -237,7 +238,11 @@ struct ssw_context {
     int launch_class_static(const ssw::DeviceBatch &db, const ssw::ClassSpec &cls);
     int launch_pass(const ssw::DeviceBatch &db, const ssw::ClassSpec &cls, const ssw_tuning &t, bool plain,
                     const ssw::PassIO &io);
-    int solve_device(const ssw::DeviceBatch &db, const uint8_t *host_n, bool plain, int64_t *violation_out);
+    // host_n: the sizes in host memory, or null for a uniform batch or when
+    // size_hist (counts per n) is given; then the sizes are copied from the
+    // device only where a route needs them one by one
+    int solve_device(const ssw::DeviceBatch &db, const uint8_t *host_n, bool plain, int64_t *violation_out,
+                     const int64_t *size_hist = nullptr);
     int solve_host_packed(const ssw_batch *b, int mode, int64_t *violation_out);
     // host staging for the int32 entry points: pinned, grown on demand
     void *h_stage = nullptr;

@@ -30,6 +30,7 @@ size_t batch_scratch_bytes(int group_lanes);
 size_t batch_group_lds(int variant, uint32_t bitmap_words, uint32_t arena_cap);
 size_t batch_block_lds(int variant, int group_lanes, int groups, uint32_t bitmap_words, uint32_t arena_cap);
 hipError_t fill_u32(uint32_t *p, uint64_t words, uint32_t value, hipStream_t st);
+hipError_t count_sizes(const uint8_t *n_per, int64_t count, unsigned long long *hist, hipStream_t st);
 hipError_t init_queues(uint32_t *queues, int64_t count, hipStream_t st, bool empty);
 hipError_t init_queues3(uint32_t *q0, uint32_t *q1, uint32_t *q2, int64_t count, uint32_t empty_mask,
                         hipStream_t st);
@@ -637,7 +638,16 @@ int ssw_context::launch_pass(const DeviceBatch &db, const ClassSpec &cls, const 
 
 // Splits the batch into launch classes: (thread-group width, bitmap in LDS).
 int ssw_context::solve_device(const DeviceBatch &db, const uint8_t *host_n, bool plain,
-                              int64_t *violation_out) {
+                              int64_t *violation_out, const int64_t *size_hist) {
+    // the sizes one by one, copied from the device when a route needs them
+    std::vector<uint8_t> fetched_n;
+    auto need_host_n = [&]() -> int {
+        if (host_n || db.uniform_n > 0) return SSW_OK;
+        fetched_n.resize((size_t)db.count);
+        HIP_TRY(hipMemcpy(fetched_n.data(), db.n_per, (size_t)db.count, hipMemcpyDeviceToHost), "D2H n_per");
+        host_n = fetched_n.data();
+        return SSW_OK;
+    };
     launches = 0;
     ++call_gen;
     int rc = ensure(deferred, 4ull * std::max<int64_t>(1, db.count), "deferred list");
@@ -656,7 +666,18 @@ int ssw_context::solve_device(const DeviceBatch &db, const uint8_t *host_n, bool
     std::vector<int64_t> hist(SSW_MAX_N + 1, 0);
     if (db.uniform_n > 0) {
         hist[db.uniform_n] = db.count;
+    } else if (!host_n && size_hist) {
+        nmin = SSW_MAX_N;
+        nmaxall = 1;
+        for (int n = 1; n <= SSW_MAX_N; ++n) {
+            hist[n] = size_hist[n];
+            if (hist[n]) {
+                nmin = std::min(nmin, n);
+                nmaxall = std::max(nmaxall, n);
+            }
+        }
     } else {
+        if ((rc = need_host_n())) return rc;
         nmin = SSW_MAX_N;
         nmaxall = 1;
         for (int64_t i = 0; i < db.count; ++i) {
@@ -670,6 +691,7 @@ int ssw_context::solve_device(const DeviceBatch &db, const uint8_t *host_n, bool
 
     std::vector<uint32_t> host_order;
     if (plain) {
+        if ((rc = need_host_n())) return rc;
         // one launch per automaton, with the LDS bitmap whenever n <= 16
         for (int n = 1; n <= SSW_MAX_N; ++n) {
             if (!hist[n]) continue;
@@ -706,9 +728,13 @@ int ssw_context::solve_device(const DeviceBatch &db, const uint8_t *host_n, bool
             else
                 classes[std::make_tuple(lds ? 0 : tune[n].group_lanes, lds ? 1 : 0, lds ? n : 0)].push_back(n);
         }
-        const bool single_class = classes.size() == 1 && db.uniform_n > 0;
+        // one class in index order: a uniform batch, or the merged sizes
+        // (the two-family launch's per-automaton cost varies little with n,
+        // and its long searches are handed on, so no size order is needed)
+        const bool single_class = classes.size() == 1 && (db.uniform_n > 0 || merged);
         std::vector<int64_t> class_start;
         if (!single_class) {
+            if ((rc = need_host_n())) return rc;
             rc = ensure(order, 4ull * db.count, "order");
             if (rc) return rc;
             // counting sort of the automata into class order, n descending
@@ -808,6 +834,7 @@ int ssw_context::solve_device(const DeviceBatch &db, const uint8_t *host_n, bool
         std::vector<uint32_t> idx(ndef);
         HIP_TRY(hipMemcpy(idx.data(), deferred.p, 4ull * ndef, hipMemcpyDeviceToHost), "D2H deferred");
         deferred_total += ndef;
+        if ((rc = need_host_n())) return rc;
         rc = large.solve_indices(*this, db, host_n, idx);
         if (rc) return rc;
         HIP_TRY(hipMemcpyAsync(&viol, (uint8_t *)ctrs.p + kCtrViolation, 8, hipMemcpyDeviceToHost, stream), "D2H");
@@ -992,7 +1019,20 @@ int ssw_context::solve_run(const ssw_batch *b, int32_t memory, int mode, int64_t
         db.witness = b->witness;
         db.witness_stride = b->witness_stride;
         std::vector<uint8_t> host_n;
-        if (b->uniform_n <= 0) {
+        int64_t size_hist[32] = {};
+        bool counted = false;
+        if (b->uniform_n <= 0 && mode != kModeLarge && b->count > 0) {
+            // the sizes counted on the device: no copy of n_per and no pass
+            // over it here unless a size is out of range (then the copy
+            // below names the first such automaton)
+            rc = ensure(sizes, 32 * sizeof(unsigned long long), "size counts");
+            if (rc) return rc;
+            HIP_TRY(count_sizes(b->n_per, b->count, (unsigned long long *)sizes.p, stream), "count sizes");
+            HIP_TRY(hipMemcpyAsync(size_hist, sizes.p, sizeof size_hist, hipMemcpyDeviceToHost, stream), "D2H sizes");
+            HIP_TRY(hipStreamSynchronize(stream), "hipStreamSynchronize");
+            counted = size_hist[0] == 0 && size_hist[31] == 0;
+        }
+        if (b->uniform_n <= 0 && !counted) {
             host_n.resize(b->count);
             hipError_t e = hipMemcpy(host_n.data(), b->n_per, b->count, hipMemcpyDeviceToHost);
             if (e != hipSuccess) return hip_fail(e, "D2H n_per");
@@ -1000,9 +1040,9 @@ int ssw_context::solve_run(const ssw_batch *b, int32_t memory, int mode, int64_t
                 if (host_n[i] < 1 || host_n[i] > SSW_MAX_N)
                     return fail(SSW_ERR_ARG, "automaton %lld: n=%d out of [1,30]", (long long)i, (int)host_n[i]);
         }
-        const uint8_t *hn = b->uniform_n > 0 ? nullptr : host_n.data();
+        const uint8_t *hn = b->uniform_n > 0 || counted ? nullptr : host_n.data();
         rc = mode == kModeLarge ? large.solve_all(*this, db, hn, violation_out)
-                                : solve_device(db, hn, mode == kModePlain, violation_out);
+                                : solve_device(db, hn, mode == kModePlain, violation_out, counted ? size_hist : nullptr);
     } else {
         rc = fail(SSW_ERR_ARG, "memory kind %d", memory);
     }
