@@ -446,15 +446,16 @@ __global__ void init_queues_kernel(uint32_t *q0, uint32_t *q1, uint32_t *q2, int
 }
 
 // The per-solve counters: deferred count <- 0, smallest violating index <-
-// none, automata done in this solve <- 0.
+// none (either may be null), automata done in this solve <- 0 in every copy
+// of the progress counters, the launch statistics <- 0.
 __global__ void reset_counters_kernel(uint32_t *deferred, unsigned long long *violation,
-                                      unsigned long long *call_done, unsigned long long *launch_stats,
+                                      unsigned long long *progress, unsigned long long *launch_stats,
                                       uint32_t stat_words) {
     if (threadIdx.x == 0) {
-        *deferred = 0u;
-        *violation = ~0ull;
-        *call_done = 0ull;
+        if (deferred) *deferred = 0u;
+        if (violation) *violation = ~0ull;
     }
+    for (uint32_t k = threadIdx.x; k < kCounterSlots; k += blockDim.x) progress[(size_t)k * kSlotWords + 2] = 0ull;
     for (uint32_t k = threadIdx.x; k < stat_words; k += blockDim.x) launch_stats[k] = 0ull;
 }
 
@@ -961,15 +962,18 @@ struct GroupSolver {
 
     __device__ void flush() {
         if (g.rank() == 0) {
+            const uint32_t slot = blockIdx.x & (kCounterSlots - 1u);
             if (done_local) {
-                atomicAdd(&P.progress[0], (unsigned long long)done_local);
-                atomicAdd(&P.progress[1], (unsigned long long)subsets_local);
-                atomicAdd(&P.progress[2], (unsigned long long)done_local);
+                unsigned long long *pg = P.progress + (size_t)slot * kSlotWords;
+                atomicAdd(&pg[0], (unsigned long long)done_local);
+                atomicAdd(&pg[1], (unsigned long long)subsets_local);
+                atomicAdd(&pg[2], (unsigned long long)done_local);
             }
             if (P.launch_stats && (done_local || expanded_local)) {
-                atomicAdd(&P.launch_stats[0], (unsigned long long)done_local);
-                atomicAdd(&P.launch_stats[1], (unsigned long long)subsets_local);
-                atomicAdd(&P.launch_stats[2], (unsigned long long)expanded_local);
+                unsigned long long *ls = P.launch_stats + (size_t)slot * kStatWords;
+                atomicAdd(&ls[0], (unsigned long long)done_local);
+                atomicAdd(&ls[1], (unsigned long long)subsets_local);
+                atomicAdd(&ls[2], (unsigned long long)expanded_local);
             }
         }
         done_local = 0;
@@ -2121,15 +2125,18 @@ struct SmallSolver {
 
     __device__ void flush() {
         if (g.rank() == 0) {
+            const uint32_t slot = blockIdx.x & (kCounterSlots - 1u);
             if (done_local) {
-                atomicAdd(&P.progress[0], (unsigned long long)done_local);
-                atomicAdd(&P.progress[1], (unsigned long long)subsets_local);
-                atomicAdd(&P.progress[2], (unsigned long long)done_local);
+                unsigned long long *pg = P.progress + (size_t)slot * kSlotWords;
+                atomicAdd(&pg[0], (unsigned long long)done_local);
+                atomicAdd(&pg[1], (unsigned long long)subsets_local);
+                atomicAdd(&pg[2], (unsigned long long)done_local);
             }
             if (P.launch_stats && (done_local || expanded_local)) {
-                atomicAdd(&P.launch_stats[0], (unsigned long long)done_local);
-                atomicAdd(&P.launch_stats[1], (unsigned long long)subsets_local);
-                atomicAdd(&P.launch_stats[2], (unsigned long long)expanded_local);
+                unsigned long long *ls = P.launch_stats + (size_t)slot * kStatWords;
+                atomicAdd(&ls[0], (unsigned long long)done_local);
+                atomicAdd(&ls[1], (unsigned long long)subsets_local);
+                atomicAdd(&ls[2], (unsigned long long)expanded_local);
             }
         }
         done_local = 0;
@@ -2307,9 +2314,9 @@ hipError_t init_queues(uint32_t *queues, int64_t count, hipStream_t st, bool emp
     return init_queues3(queues, nullptr, nullptr, count, empty ? 1u : 0u, st);
 }
 
-hipError_t reset_counters(uint32_t *deferred, unsigned long long *violation, unsigned long long *call_done,
+hipError_t reset_counters(uint32_t *deferred, unsigned long long *violation, unsigned long long *progress,
                           unsigned long long *launch_stats, uint32_t stat_words, hipStream_t st) {
-    hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, st, deferred, violation, call_done,
+    hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(256), 0, st, deferred, violation, progress,
                        launch_stats, stat_words);
     return hipGetLastError();
 }

@@ -24,8 +24,7 @@ enum SolveMode { kModeBatch = 0, kModePlain = 1, kModeLarge = 2 };
 constexpr size_t kCtrWork = 0;         // u32 work-queue head
 constexpr size_t kCtrDeferred = 4;     // u32 deferred count
 constexpr size_t kCtrViolation = 8;    // u64 smallest violating index
-constexpr size_t kCtrProgress = 16;    // u64[3] automata done, subsets expanded (all solves),
-                                       // automata done in the running solve
+// (progress counters: DevBuf prog, kCounterSlots copies)
 constexpr size_t kCtrLarge = 64;       // device-wide engine scratch counters
 constexpr size_t kCtrBytes = 256;
 
@@ -190,7 +189,9 @@ struct ssw_context {
     // per-launch statistics of the last batched solve: kMaxLaunchRecords
     // triples of u64 counters (zeroed with the solve's counters), the events
     // around each launch and its shape
-    ssw::DevBuf lstats;
+    ssw::DevBuf lstats;  // launch statistics: kMaxLaunchRecords x kCounterSlots copies of kStatWords
+    ssw::DevBuf prog;    // progress counters: kCounterSlots copies of [0] done, [1] subsets, [2] done this solve
+    static constexpr uint32_t kLaunchStatWords = ssw::kMaxLaunchRecords * ssw::kCounterSlots * ssw::kStatWords;
     hipEvent_t lev[2 * ssw::kMaxLaunchRecords] = {};
     std::vector<ssw::LaunchRecord> records;
     ssw::DevBuf d_delta, d_nper, d_off, d_len, d_sync, d_exp, d_wit;

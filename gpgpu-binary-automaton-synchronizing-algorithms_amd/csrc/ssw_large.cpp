@@ -23,6 +23,8 @@ hipError_t large_launch_back(const LargeInstance *inst, int slot, int j, int L, 
 hipError_t large_launch_fwd(const LargeInstance *inst, int slot, int L, const uint32_t *g0, const uint32_t *g1,
                             uint8_t *out, int stride, hipStream_t st);
 hipError_t fill_u32(uint32_t *p, uint64_t words, uint32_t value, hipStream_t st);
+hipError_t reset_counters(uint32_t *deferred, unsigned long long *violation, unsigned long long *progress,
+                          unsigned long long *launch_stats, uint32_t stat_words, hipStream_t st);
 
 #define LG_TRY(expr, what)                                  \
     do {                                                    \
@@ -321,7 +323,7 @@ int LargeEngine::solve_indices(ssw_context &ctx, const DeviceBatch &db, const ui
     if (rc) return rc;
     uint8_t *c = (uint8_t *)ctx.ctrs.p;
     rc = set_outputs(ctx, db.length, db.sync, db.expanded, (unsigned long long *)(c + kCtrViolation),
-                     (unsigned long long *)(c + kCtrProgress));
+                     (unsigned long long *)ctx.prog.p);
     if (rc) return rc;
     const bool append = db.witness != nullptr;
     const uint64_t budget = ctx.budget / 2;
@@ -399,7 +401,7 @@ int LargeEngine::solve_all(ssw_context &ctx, const DeviceBatch &db, const uint8_
         hipSuccess)
         return ctx.hip_fail(e, "fill");
     // automata done in this solve <- 0
-    if ((e = fill_u32((uint32_t *)((uint8_t *)ctx.ctrs.p + kCtrProgress + 16), 2, 0u, ctx.stream)) != hipSuccess)
+    if ((e = reset_counters(nullptr, nullptr, (unsigned long long *)ctx.prog.p, nullptr, 0, ctx.stream)) != hipSuccess)
         return ctx.hip_fail(e, "fill");
     std::vector<uint32_t> idx(db.count);
     for (int64_t i = 0; i < db.count; ++i) idx[i] = (uint32_t)i;

@@ -34,7 +34,7 @@ hipError_t count_sizes(const uint8_t *n_per, int64_t count, unsigned long long *
 hipError_t init_queues(uint32_t *queues, int64_t count, hipStream_t st, bool empty);
 hipError_t init_queues3(uint32_t *q0, uint32_t *q1, uint32_t *q2, int64_t count, uint32_t empty_mask,
                         hipStream_t st);
-hipError_t reset_counters(uint32_t *deferred, unsigned long long *violation, unsigned long long *call_done,
+hipError_t reset_counters(uint32_t *deferred, unsigned long long *violation, unsigned long long *progress,
                           unsigned long long *launch_stats, uint32_t stat_words, hipStream_t st);
 hipError_t noalloc_scratch(int device, void **out);
 std::mutex &noalloc_call_lock(int device);
@@ -119,10 +119,13 @@ int ssw_context::open(int dev, uint64_t mem_budget) {
     if (rc) return rc;
     rc = ensure(queues, kQueueBytes, "work queues");
     if (rc) return rc;
-    rc = ensure(lstats, 24u * kMaxLaunchRecords, "launch statistics");
+    rc = ensure(lstats, 8u * kLaunchStatWords, "launch statistics");
+    if (rc) return rc;
+    rc = ensure(prog, kProgressBytes, "progress counters");
     if (rc) return rc;
     for (int k = 0; k < 2 * kMaxLaunchRecords; ++k) HIP_TRY(hipEventCreate(&lev[k]), "hipEventCreate");
-    HIP_TRY(hipMemsetAsync(lstats.p, 0, 24u * kMaxLaunchRecords, stream), "hipMemset");
+    HIP_TRY(hipMemsetAsync(lstats.p, 0, 8u * kLaunchStatWords, stream), "hipMemset");
+    HIP_TRY(hipMemsetAsync(prog.p, 0, kProgressBytes, stream), "hipMemset");
     HIP_TRY(hipMemsetAsync(ctrs.p, 0, kCtrBytes, stream), "hipMemset");
     HIP_TRY(hipStreamSynchronize(stream), "hipStreamSynchronize");
     // the allocation-free entry point's scratch on this device (its own
@@ -135,7 +138,7 @@ int ssw_context::open(int dev, uint64_t mem_budget) {
 void ssw_context::close() {
     stop_progress_thread();
     if (device >= 0) hipSetDevice(device);
-    for (DevBuf *b : {&ctrs, &queues, &deferred, &order, &spill, &lstats, &d_delta, &d_nper, &d_off, &d_len,
+    for (DevBuf *b : {&ctrs, &queues, &deferred, &order, &spill, &lstats, &prog, &sizes, &d_delta, &d_nper, &d_off, &d_len,
                       &d_sync, &d_exp, &d_wit})
         if (b->p) hipFree(b->p);
     for (int k = 0; k < 2 * kMaxLaunchRecords; ++k)
@@ -452,7 +455,7 @@ int ssw_context::launch_pass(const DeviceBatch &db, const ClassSpec &cls, const 
     hipStream_t st = cls.stream ? cls.stream : stream;
     P.deferred_count = (unsigned int *)(c + kCtrDeferred);
     P.violation = (unsigned long long *)(c + kCtrViolation);
-    P.progress = (unsigned long long *)(c + kCtrProgress);
+    P.progress = (unsigned long long *)prog.p;
     P.deferred_list = (uint32_t *)deferred.p;
     P.chunk = (uint32_t)std::max(1, t.chunk);
     P.lds_n = use_lds ? nmax : 0;
@@ -616,7 +619,7 @@ int ssw_context::launch_pass(const DeviceBatch &db, const ClassSpec &cls, const 
             r.byte_tab = byte_tab;
             r.witness = want_wit;
             records.push_back(r);
-            P.launch_stats = (unsigned long long *)lstats.p + 3 * rec;
+            P.launch_stats = (unsigned long long *)lstats.p + (size_t)rec * kCounterSlots * kStatWords;
             HIP_TRY(hipEventRecord(lev[2 * rec], st), "hipEventRecord");
         }
         HIP_TRY(launch_batch(variant, G, P, grid, block, lds, st), "batch kernel launch");
@@ -657,8 +660,7 @@ int ssw_context::solve_device(const DeviceBatch &db, const uint8_t *host_n, bool
     records.clear();
     HIP_TRY(reset_counters((uint32_t *)((uint8_t *)ctrs.p + kCtrDeferred),
                            (unsigned long long *)((uint8_t *)ctrs.p + kCtrViolation),
-                           (unsigned long long *)((uint8_t *)ctrs.p + kCtrProgress) + 2,
-                           (unsigned long long *)lstats.p, 3u * kMaxLaunchRecords, stream),
+                           (unsigned long long *)prog.p, (unsigned long long *)lstats.p, kLaunchStatWords, stream),
             "reset counters");
     HIP_TRY(hipEventRecord(ev0, stream), "hipEventRecord");
 
@@ -1100,8 +1102,8 @@ int ssw_context::solve_noalloc(const ssw_batch *b, int64_t *violation_out) {
     records.clear();
     hipError_t e = reset_counters((uint32_t *)((uint8_t *)ctrs.p + kCtrDeferred),
                                   (unsigned long long *)((uint8_t *)ctrs.p + kCtrViolation),
-                                  (unsigned long long *)((uint8_t *)ctrs.p + kCtrProgress) + 2,
-                                  (unsigned long long *)lstats.p, 3u * kMaxLaunchRecords, stream);
+                                  (unsigned long long *)prog.p, (unsigned long long *)lstats.p, kLaunchStatWords,
+                                  stream);
     if (e == hipSuccess) e = hipEventRecord(ev0, stream);
     if (e != hipSuccess) rc = hip_fail(e, "set-up");
     if (rc == SSW_OK) rc = launch_class(db, cls, false);
@@ -1262,9 +1264,15 @@ int ssw_context::launch_stats(ssw_launch_stat *out, int32_t cap, int32_t *count)
     *count = (int32_t)records.size();
     if (records.empty() || cap == 0) return SSW_OK;
     HIP_TRY(hipSetDevice(device), "hipSetDevice");
-    unsigned long long c[3 * kMaxLaunchRecords];
-    HIP_TRY(hipMemcpyAsync(c, lstats.p, sizeof c, hipMemcpyDeviceToHost, stream), "D2H launch statistics");
+    std::vector<unsigned long long> raw(kLaunchStatWords);
+    HIP_TRY(hipMemcpyAsync(raw.data(), lstats.p, 8u * kLaunchStatWords, hipMemcpyDeviceToHost, stream),
+            "D2H launch statistics");
     HIP_TRY(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    // the copies of each record summed
+    std::vector<unsigned long long> c(3 * kMaxLaunchRecords, 0ull);
+    for (uint32_t k = 0; k < (uint32_t)kMaxLaunchRecords; ++k)
+        for (uint32_t s = 0; s < kCounterSlots; ++s)
+            for (uint32_t j = 0; j < 3; ++j) c[3 * k + j] += raw[((size_t)k * kCounterSlots + s) * kStatWords + j];
     for (size_t k = 0; k < records.size() && (int32_t)k < cap; ++k) {
         const LaunchRecord &r = records[k];
         ssw_launch_stat &o = out[k];
@@ -1307,12 +1315,14 @@ int ssw_context::read_progress(ssw_progress *out) {
     // one the done-in-this-solve counter belongs to
     const uint64_t g0 = solve_gen.load();
     const uint64_t cnt = call_count.load();
-    unsigned long long v[3] = {0, 0, 0};
+    std::vector<unsigned long long> raw(kProgressBytes / 8);
     hipError_t e = hipSetDevice(device);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(v, (uint8_t *)ctrs.p + kCtrProgress, sizeof v, hipMemcpyDeviceToHost, monitor);
+    if (e == hipSuccess) e = hipMemcpyAsync(raw.data(), prog.p, kProgressBytes, hipMemcpyDeviceToHost, monitor);
     if (e == hipSuccess) e = hipStreamSynchronize(monitor);
     if (e != hipSuccess) return SSW_ERR_DEVICE;
+    unsigned long long v[3] = {0, 0, 0};
+    for (uint32_t s = 0; s < kCounterSlots; ++s)
+        for (uint32_t j = 0; j < 3; ++j) v[j] += raw[(size_t)s * kSlotWords + j];
     const bool running = (g0 & 1u) && solve_gen.load() == g0;
     out->automata_done = v[0] + large_done.load();
     out->subsets_expanded = v[1] + large_subsets.load();
