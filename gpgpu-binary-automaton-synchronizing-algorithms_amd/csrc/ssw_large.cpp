@@ -416,7 +416,8 @@ int LargeEngine::solve_all(ssw_context &ctx, const DeviceBatch &db, const uint8_
     ctx.last_ms = ms;
     ctx.last_launches = (int64_t)(ctx.total_launches.load() - before);
     unsigned long long viol = ~0ull;
-    LG_TRY(hipMemcpy(&viol, (uint8_t *)ctx.ctrs.p + kCtrViolation, 8, hipMemcpyDeviceToHost), "D2H");
+    LG_TRY(hipMemcpyAsync(&viol, (uint8_t *)ctx.ctrs.p + kCtrViolation, 8, hipMemcpyDeviceToHost, ctx.stream), "D2H");
+    LG_TRY(hipStreamSynchronize(ctx.stream), "sync");
     if (violation_out) *violation_out = viol == ~0ull ? -1 : (int64_t)viol;
     return SSW_OK;
 }

@@ -535,11 +535,13 @@ int ssw_context::launch_pass(const DeviceBatch &db, const ClassSpec &cls, const 
             P.hist_off = P.gbitmap_off = P.good_off = P.lvl_off = off;
         }
     } else {
-        // every subset fits the HBM arena when n <= 20; larger automata whose
+        // every subset fits the HBM arena when n <= 20 (with the step's
+        // margin of two images per entry taken); larger automata whose
         // frontier outgrows it are deferred to the device-wide path
         uint64_t garena_cap;
+        const uint64_t margin = 2ull * G * (variant >= 11 ? 4u : 1u);
         if (use_lds || nmax <= 20)
-            garena_cap = all_subsets + 2u * G > arena_lds ? all_subsets + 2u * G - arena_lds : 0;
+            garena_cap = all_subsets + margin > arena_lds ? all_subsets + margin - arena_lds : 0;
         else
             garena_cap = plain ? (1ull << 22) : (1ull << 18);
         const uint64_t gbm_words = use_lds ? 0 : std::max<uint64_t>(1, all_subsets / 32);
@@ -647,7 +649,9 @@ int ssw_context::solve_device(const DeviceBatch &db, const uint8_t *host_n, bool
     auto need_host_n = [&]() -> int {
         if (host_n || db.uniform_n > 0) return SSW_OK;
         fetched_n.resize((size_t)db.count);
-        HIP_TRY(hipMemcpy(fetched_n.data(), db.n_per, (size_t)db.count, hipMemcpyDeviceToHost), "D2H n_per");
+        HIP_TRY(hipMemcpyAsync(fetched_n.data(), db.n_per, (size_t)db.count, hipMemcpyDeviceToHost, stream),
+                "D2H n_per");
+        HIP_TRY(hipStreamSynchronize(stream), "hipStreamSynchronize");
         host_n = fetched_n.data();
         return SSW_OK;
     };
@@ -834,7 +838,8 @@ int ssw_context::solve_device(const DeviceBatch &db, const uint8_t *host_n, bool
     last_launches = launches;
     if (ndef) {
         std::vector<uint32_t> idx(ndef);
-        HIP_TRY(hipMemcpy(idx.data(), deferred.p, 4ull * ndef, hipMemcpyDeviceToHost), "D2H deferred");
+        HIP_TRY(hipMemcpyAsync(idx.data(), deferred.p, 4ull * ndef, hipMemcpyDeviceToHost, stream), "D2H deferred");
+        HIP_TRY(hipStreamSynchronize(stream), "hipStreamSynchronize");
         deferred_total += ndef;
         if ((rc = need_host_n())) return rc;
         rc = large.solve_indices(*this, db, host_n, idx);
@@ -1036,7 +1041,8 @@ int ssw_context::solve_run(const ssw_batch *b, int32_t memory, int mode, int64_t
         }
         if (b->uniform_n <= 0 && !counted) {
             host_n.resize(b->count);
-            hipError_t e = hipMemcpy(host_n.data(), b->n_per, b->count, hipMemcpyDeviceToHost);
+            hipError_t e = hipMemcpyAsync(host_n.data(), b->n_per, b->count, hipMemcpyDeviceToHost, stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(stream);
             if (e != hipSuccess) return hip_fail(e, "D2H n_per");
             for (int64_t i = 0; i < b->count; ++i)
                 if (host_n[i] < 1 || host_n[i] > SSW_MAX_N)

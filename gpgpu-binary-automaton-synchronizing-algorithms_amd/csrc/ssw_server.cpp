@@ -101,8 +101,13 @@ struct HipLarge final : LargeBackend {
                 break;
             }
         if (free_idx < 0) return SSW_OK;
-        if (hipMemcpy(d_delta + 64ull * free_idx, delta, 2 * (size_t)n, hipMemcpyHostToDevice) != hipSuccess) {
-            *err = "staging the transitions failed";
+        // on the engine's own stream: a copy on the legacy stream would wait
+        // on (and be refused beside) other contexts' streams in the process
+        hipError_t e = hipMemcpyAsync(d_delta + 64ull * free_idx, delta, 2 * (size_t)n, hipMemcpyHostToDevice,
+                                      ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) {
+            *err = std::string("staging the transitions failed: ") + hipGetErrorName(e) + " (" + hipGetErrorString(e) + ")";
             return SSW_ERR_DEVICE;
         }
         const int rc = E.admit(*ctx, 0, n, d_delta + 64ull * free_idx, nullptr, -1, false, witness, budget, slot);
@@ -136,8 +141,9 @@ struct HipLarge final : LargeBackend {
             f.expanded = (int64_t)I.expanded;
             if (witness && I.status == kLFound) {
                 if (E.witness(*ctx, s, d_wit + (size_t)kStride * s, kStride) == SSW_OK &&
-                    hipMemcpy(hwit.data(), d_wit + (size_t)kStride * s, kStride, hipMemcpyDeviceToHost) ==
-                        hipSuccess) {
+                    hipMemcpyAsync(hwit.data(), d_wit + (size_t)kStride * s, kStride, hipMemcpyDeviceToHost,
+                                   ctx->stream) == hipSuccess &&
+                    hipStreamSynchronize(ctx->stream) == hipSuccess) {
                     f.witness_valid = true;
                     std::memcpy(f.witness, hwit.data(), kStride);
                 }
@@ -231,6 +237,11 @@ int ssw_server_wait(ssw_server *s, uint64_t ticket, int32_t timeout_ms, ssw_resu
 int ssw_server_stats_get(ssw_server *s, ssw_server_stats *out) {
     if (!s) return SSW_ERR_ARG;
     return s->core->stats(out);
+}
+
+const char *ssw_server_last_error(ssw_server *s) {
+    if (!s) return "";
+    return s->core->last_error();
 }
 
 int ssw_server_destroy(ssw_server *s) {

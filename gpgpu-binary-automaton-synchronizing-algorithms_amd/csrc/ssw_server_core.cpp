@@ -284,6 +284,11 @@ int ServerCore::wait(uint64_t ticket, int32_t timeout_ms, ssw_result *out) {
     return rc;
 }
 
+const char *ServerCore::last_error() {
+    std::lock_guard<std::mutex> lk(mu_);
+    return error_ != SSW_OK ? error_msg_.c_str() : "";
+}
+
 int ServerCore::stats(ssw_server_stats *out) {
     if (!out) return SSW_ERR_ARG;
     std::lock_guard<std::mutex> lk(mu_);

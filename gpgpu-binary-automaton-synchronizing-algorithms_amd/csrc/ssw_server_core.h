@@ -90,6 +90,9 @@ public:
     int poll(uint64_t ticket, ssw_result *out);
     int wait(uint64_t ticket, int32_t timeout_ms, ssw_result *out);
     int stats(ssw_server_stats *out);
+    // the message of the error that stopped the loop ("" if none); stable
+    // once set (it is set once)
+    const char *last_error();
     // Stops taking submissions, lets the engines finish every request already
     // submitted, joins them, wakes the callers still blocked in wait (they
     // return their results or SSW_ERR_STATE) and returns once none is left

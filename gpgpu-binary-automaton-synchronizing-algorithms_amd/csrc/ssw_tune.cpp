@@ -182,7 +182,9 @@ int ssw_autotune(ssw_context *ctx, int32_t n_lo, int32_t n_hi, int64_t samples, 
         int rc;
         if ((rc = ctx->ensure(ctx->d_delta, packed.size(), "delta"))) return rc;
         if ((rc = ctx->ensure(ctx->d_len, 4ull * count, "length"))) return rc;
-        if (hipMemcpy(ctx->d_delta.p, packed.data(), packed.size(), hipMemcpyHostToDevice) != hipSuccess)
+        if (hipMemcpyAsync(ctx->d_delta.p, packed.data(), packed.size(), hipMemcpyHostToDevice, ctx->stream) !=
+                hipSuccess ||
+            hipStreamSynchronize(ctx->stream) != hipSuccess)
             return ctx->fail(SSW_ERR_DEVICE, "H2D");
         DeviceBatch db;
         db.count = count;

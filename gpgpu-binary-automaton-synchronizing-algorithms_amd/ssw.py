@@ -40,6 +40,7 @@ EXPORTS = [
     "ssw_generate_cerny", "ssw_generate_dotnet", "ssw_server_create", "ssw_server_submit",
     "ssw_server_poll", "ssw_server_wait", "ssw_server_stats_get", "ssw_server_destroy",
     "ssw_measure_atomics", "ssw_solve_batch_noalloc", "ssw_last_launch_stats", "ssw_set_verify_prefilter",
+    "ssw_server_last_error",
 ]
 
 
@@ -154,6 +155,7 @@ def lib():
             "ssw_server_wait": ([vp, u64, i32, ctypes.POINTER(Result)], ctypes.c_int),
             "ssw_server_stats_get": ([vp, ctypes.POINTER(ServerStats)], ctypes.c_int),
             "ssw_server_destroy": ([vp], ctypes.c_int),
+            "ssw_server_last_error": ([vp], ctypes.c_char_p),
             "ssw_measure_atomics": ([vp, i32, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
             "ssw_last_launch_stats": ([vp, ctypes.POINTER(LaunchStat), i32, ctypes.POINTER(i32)], ctypes.c_int),
         }
@@ -481,7 +483,7 @@ class Server:
         t = ctypes.c_uint64()
         rc = self._L.ssw_server_submit(self.h, n, _ptr(a), _ptr(b), ctypes.byref(t))
         if rc:
-            raise SswError(rc, "submit")
+            raise SswError(rc, self._why("submit"))
         return t.value
 
     @staticmethod
@@ -496,21 +498,25 @@ class Server:
         if rc == SSW_PENDING:
             return None
         if rc:
-            raise SswError(rc, "poll")
+            raise SswError(rc, self._why("poll"))
         return self._res(r)
 
     def wait(self, ticket, timeout_ms=60000):
         r = Result()
         rc = self._L.ssw_server_wait(self.h, ticket, timeout_ms, ctypes.byref(r))
         if rc:
-            raise SswError(rc, "wait")
+            raise SswError(rc, self._why("wait"))
         return self._res(r)
+
+    def _why(self, what):
+        msg = self._L.ssw_server_last_error(self.h).decode()
+        return f"{what}: {msg}" if msg else what
 
     def stats(self):
         s = ServerStats()
         rc = self._L.ssw_server_stats_get(self.h, ctypes.byref(s))
         if rc:
-            raise SswError(rc, "stats")
+            raise SswError(rc, self._why("stats"))
         return {k: getattr(s, k) for k, _ in ServerStats._fields_}
 
     def close(self):

@@ -342,6 +342,10 @@ int ssw_server_poll(ssw_server *s, uint64_t ticket, ssw_result *out);
 int ssw_server_wait(ssw_server *s, uint64_t ticket, int32_t timeout_ms, ssw_result *out);
 /* Percentiles over the last 2^20 completed requests. */
 int ssw_server_stats_get(ssw_server *s, ssw_server_stats *out);
+/* Why the server stopped on an error (an engine's HIP failure: then every
+   pending and later call returns that error), or "" while it has not.  The
+   text stays valid until ssw_server_destroy. */
+const char *ssw_server_last_error(ssw_server *s);
 /* Stops taking submissions, finishes every request already submitted, wakes
    the callers blocked in ssw_server_wait (they return their result, or
    SSW_ERR_STATE) and returns once none is left inside the server's calls;

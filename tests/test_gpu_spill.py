@@ -236,6 +236,35 @@ def test_two_family_encoding(ctx):
     assert sum(x["automata"] for x in fam) >= 0.6 * len(cases), [(x["n"], x["automata"]) for x in fam]
 
 
+def test_general_pass_keeps_n20_on_the_batched_path(ctx):
+    """What the two-family launch hands on runs in the general solver with
+    four entries per lane; its HBM arena holds every subset of n <= 20 with
+    that step's margin, so Cerny C_20 (2^20 - 21 subsets) finishes there and
+    nothing is deferred to the device-wide path.  Lengths and expanded counts
+    by the closed forms and the oracle."""
+    rng = np.random.default_rng(77)
+    ns, As, Bs = [], [], []
+    for conv in (0, 1):
+        a, b = ssw.generate_cerny(20, conv)
+        ns.append(20)
+        As.append(np.asarray(a, np.int32))
+        Bs.append(np.asarray(b, np.int32))
+    for n in (17, 18, 19, 20) * 30:
+        ns.append(n)
+        As.append(rng.integers(0, n, size=n).astype(np.int32))
+        Bs.append(rng.integers(0, n, size=n).astype(np.int32))
+    before = ctx.progress()["deferred"]
+    r = ctx.solve_mixed(np.array(ns, np.uint8), As, Bs)
+    assert ctx.progress()["deferred"] == before
+    assert int(r.length[0]) == int(r.length[1]) == 19 * 19
+    assert int(r.expanded[0]) == int(r.expanded[1]) == 2 ** 20 - 21
+    for n in (17, 18, 19, 20):
+        idx = [i for i in range(2, len(ns)) if ns[i] == n]
+        L, S, E, _ = O.solve(n, np.stack([As[i] for i in idx]), np.stack([Bs[i] for i in idx]), threads=8)
+        np.testing.assert_array_equal(r.length[idx], L, err_msg=f"n={n}")
+        np.testing.assert_array_equal(r.expanded[idx], E, err_msg=f"n={n}")
+
+
 WIT_CONFIGS = [
     (64, 128, 16, 512, 1, 4096, 4, 0),       # the committed spill chain
     (64, 0, 4, 256, 1, 2048, 4, 0),          # last launch's ring too small for C_16: HBM ring
