@@ -1270,13 +1270,13 @@ int ssw_context::launch_stats(ssw_launch_stat *out, int32_t cap, int32_t *count)
     *count = (int32_t)records.size();
     if (records.empty() || cap == 0) return SSW_OK;
     HIP_TRY(hipSetDevice(device), "hipSetDevice");
-    std::vector<unsigned long long> raw(kLaunchStatWords);
-    HIP_TRY(hipMemcpyAsync(raw.data(), lstats.p, 8u * kLaunchStatWords, hipMemcpyDeviceToHost, stream),
-            "D2H launch statistics");
+    // the records in use, their copies summed
+    const size_t used = records.size() * kCounterSlots * kStatWords;
+    std::vector<unsigned long long> raw(used);
+    HIP_TRY(hipMemcpyAsync(raw.data(), lstats.p, 8u * used, hipMemcpyDeviceToHost, stream), "D2H launch statistics");
     HIP_TRY(hipStreamSynchronize(stream), "hipStreamSynchronize");
-    // the copies of each record summed
-    std::vector<unsigned long long> c(3 * kMaxLaunchRecords, 0ull);
-    for (uint32_t k = 0; k < (uint32_t)kMaxLaunchRecords; ++k)
+    std::vector<unsigned long long> c(3 * records.size(), 0ull);
+    for (uint32_t k = 0; k < (uint32_t)records.size(); ++k)
         for (uint32_t s = 0; s < kCounterSlots; ++s)
             for (uint32_t j = 0; j < 3; ++j) c[3 * k + j] += raw[((size_t)k * kCounterSlots + s) * kStatWords + j];
     for (size_t k = 0; k < records.size() && (int32_t)k < cap; ++k) {
