@@ -239,6 +239,11 @@ int ssw_server_stats_get(ssw_server *s, ssw_server_stats *out) {
     return s->core->stats(out);
 }
 
+int ssw_server_cancel(ssw_server *s, uint64_t ticket) {
+    if (!s) return SSW_ERR_ARG;
+    return s->core->cancel(ticket);
+}
+
 const char *ssw_server_last_error(ssw_server *s) {
     if (!s) return "";
     return s->core->last_error();

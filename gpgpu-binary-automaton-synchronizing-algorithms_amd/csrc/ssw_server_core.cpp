@@ -259,7 +259,7 @@ int ServerCore::wait(uint64_t ticket, int32_t timeout_ms, ssw_result *out) {
     // shutdown() leave it to the last waiter
     ++r->waiters;
     ++waiting_;
-    const auto ready = [&] { return r->done || r->collected || error_ != SSW_OK || closed_; };
+    const auto ready = [&] { return r->done || r->collected || r->cancelled || error_ != SSW_OK || closed_; };
     bool in_time = true;
     if (timeout_ms < 0)
         cv_done_.wait(lk, ready);
@@ -268,7 +268,10 @@ int ServerCore::wait(uint64_t ticket, int32_t timeout_ms, ssw_result *out) {
     --r->waiters;
     --waiting_;
     int rc;
-    if (r->collected) {
+    if (r->cancelled) {
+        rc = SSW_ERR_CANCELLED;
+        if (r->waiters == 0) delete r;
+    } else if (r->collected) {
         // another caller took the result while this one slept
         rc = SSW_ERR_NOTFOUND;
         if (r->waiters == 0) delete r;
@@ -282,6 +285,26 @@ int ServerCore::wait(uint64_t ticket, int32_t timeout_ms, ssw_result *out) {
     }
     if (closed_ && waiting_ == 0) cv_idle_.notify_all();
     return rc;
+}
+
+int ServerCore::cancel(uint64_t ticket) {
+    std::lock_guard<std::mutex> lk(mu_);
+    auto it = reqs_.find(ticket);
+    if (it == reqs_.end()) return SSW_ERR_NOTFOUND;
+    ServerRequest *r = it->second;
+    if (r->done) return SSW_ERR_STATE;
+    // only from its queue: a request an engine has taken is being solved
+    std::deque<ServerRequest *> &q = r->n <= cfg_.small_n_max ? small_q_ : large_q_;
+    const auto qi = std::find(q.begin(), q.end(), r);
+    if (qi == q.end()) return SSW_ERR_STATE;
+    q.erase(qi);
+    reqs_.erase(it);
+    r->cancelled = true;
+    if (r->waiters == 0)
+        delete r;  // else the last waiter frees it
+    else
+        cv_done_.notify_all();
+    return SSW_OK;
 }
 
 const char *ServerCore::last_error() {

@@ -33,6 +33,7 @@ struct ServerRequest {
     ServerClock::time_point t_submit, t_done;
     bool done = false;
     bool collected = false;  // the result was handed out; the ticket is gone
+    bool cancelled = false;  // taken out of its queue by cancel; the ticket is gone
     int waiters = 0;         // callers blocked in wait on this request
     ssw_result res;
 };
@@ -89,6 +90,7 @@ public:
     int submit(int32_t n, const int32_t *a, const int32_t *b, uint64_t *ticket);
     int poll(uint64_t ticket, ssw_result *out);
     int wait(uint64_t ticket, int32_t timeout_ms, ssw_result *out);
+    int cancel(uint64_t ticket);
     int stats(ssw_server_stats *out);
     // the message of the error that stopped the loop ("" if none); stable
     // once set (it is set once)

@@ -25,6 +25,7 @@ SSW_ERR_STATE = -5
 SSW_ERR_INTERNAL = -6
 SSW_ERR_TIMEOUT = -7
 SSW_ERR_NOTFOUND = -8
+SSW_ERR_CANCELLED = -9
 SSW_HOST = 0
 SSW_DEVICE = 1
 LEN_NOT_SYNC = -1
@@ -40,7 +41,7 @@ EXPORTS = [
     "ssw_generate_cerny", "ssw_generate_dotnet", "ssw_server_create", "ssw_server_submit",
     "ssw_server_poll", "ssw_server_wait", "ssw_server_stats_get", "ssw_server_destroy",
     "ssw_measure_atomics", "ssw_solve_batch_noalloc", "ssw_last_launch_stats", "ssw_set_verify_prefilter",
-    "ssw_server_last_error",
+    "ssw_server_last_error", "ssw_server_cancel",
 ]
 
 
@@ -156,6 +157,7 @@ def lib():
             "ssw_server_stats_get": ([vp, ctypes.POINTER(ServerStats)], ctypes.c_int),
             "ssw_server_destroy": ([vp], ctypes.c_int),
             "ssw_server_last_error": ([vp], ctypes.c_char_p),
+            "ssw_server_cancel": ([vp, ctypes.c_uint64], ctypes.c_int),
             "ssw_measure_atomics": ([vp, i32, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
             "ssw_last_launch_stats": ([vp, ctypes.POINTER(LaunchStat), i32, ctypes.POINTER(i32)], ctypes.c_int),
         }
@@ -507,6 +509,16 @@ class Server:
         if rc:
             raise SswError(rc, self._why("wait"))
         return self._res(r)
+
+    def cancel(self, ticket):
+        """True when the request was still queued and is cancelled (its
+        ticket released), False when it is already being solved or done."""
+        rc = self._L.ssw_server_cancel(self.h, ticket)
+        if rc == SSW_ERR_STATE:
+            return False
+        if rc:
+            raise SswError(rc, self._why("cancel"))
+        return True
 
     def _why(self, what):
         msg = self._L.ssw_server_last_error(self.h).decode()

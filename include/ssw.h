@@ -46,7 +46,8 @@ enum ssw_status {
     SSW_ERR_STATE = -5,        /* object in the wrong state (e.g. server stopped) */
     SSW_ERR_INTERNAL = -6,
     SSW_ERR_TIMEOUT = -7,
-    SSW_ERR_NOTFOUND = -8      /* unknown ticket */
+    SSW_ERR_NOTFOUND = -8,     /* unknown ticket */
+    SSW_ERR_CANCELLED = -9     /* the request was cancelled (ssw_server_cancel) */
 };
 
 /* Per-automaton length codes written by the device besides L >= 0 / -1. */
@@ -340,6 +341,12 @@ int ssw_server_poll(ssw_server *s, uint64_t ticket, ssw_result *out);
    callers wait on one ticket, one gets the result and the others
    SSW_ERR_NOTFOUND. */
 int ssw_server_wait(ssw_server *s, uint64_t ticket, int32_t timeout_ms, ssw_result *out);
+/* Cancels a request still waiting in its queue (not yet taken by an engine):
+   SSW_OK and the ticket is released -- callers blocked in ssw_server_wait on
+   it return SSW_ERR_CANCELLED, later calls with it SSW_ERR_NOTFOUND.  A
+   request already being solved, or done, is not cancelled: SSW_ERR_STATE (its
+   result stays available).  SSW_ERR_NOTFOUND for an unknown ticket. */
+int ssw_server_cancel(ssw_server *s, uint64_t ticket);
 /* Percentiles over the last 2^20 completed requests. */
 int ssw_server_stats_get(ssw_server *s, ssw_server_stats *out);
 /* Why the server stopped on an error (an engine's HIP failure: then every

@@ -11,10 +11,14 @@
 // Scenarios: many submitting/waiting/polling threads at once; two callers
 // waiting on one ticket (one result, one SSW_ERR_NOTFOUND); a timed-out wait
 // keeping the ticket; the budget check at submit agreeing with admission;
-// destroy while callers are blocked in wait and requests are still queued.
+// destroy while callers are blocked in wait and requests are still queued;
+// cancellation racing the engines, with callers blocked on the cancelled
+// tickets.
 #include "ssw_server_core.h"
 
 #include <atomic>
+#include <condition_variable>
+#include <mutex>
 #include <cstdio>
 #include <cstring>
 #include <cstdlib>
@@ -328,7 +332,137 @@ static void scenario_destroy_with_waiters() {
     }
 }
 
+// Cancel while the engines take requests: a cancel either wins (the ticket is
+// gone, its blocked waiters return SSW_ERR_CANCELLED) or finds the request
+// taken (SSW_ERR_STATE, and its result arrives); no other outcome, and every
+// request not cancelled gets the oracle's result.
+static void scenario_cancel() {
+    for (int rep = 0; rep < 4; ++rep) {
+        StubSmall s0, s1;
+        StubLarge large;
+        ServerCore core(config(10, 2, false), {&s0, &s1}, &large);
+        core.start();
+        std::mt19937 rng(90 + rep);
+        const int m = 200;
+        std::vector<Req> reqs;
+        std::vector<uint64_t> tickets;
+        for (int i = 0; i < m; ++i) {
+            reqs.push_back(make_req(rng, 3, 19));
+            uint64_t tk = 0;
+            CHECK(core.submit(reqs.back().n, reqs.back().a.data(), reqs.back().b.data(), &tk) == SSW_OK, "submit");
+            tickets.push_back(tk);
+        }
+        // a blocked waiter on every third ticket, then cancels of every
+        // other ticket from two threads
+        std::vector<int> cancel_rc(m, 1), wait_rc(m, 1);
+        std::vector<std::thread> ws;
+        for (int k = 0; k < 4; ++k)
+            ws.emplace_back([&, k] {
+                for (int i = k * 3; i < m; i += 12) {
+                    ssw_result res;
+                    wait_rc[i] = core.wait(tickets[i], 60000, &res);
+                    if (wait_rc[i] == SSW_OK) CHECK(res.length == reqs[i].want_len, "result of %d", i);
+                }
+            });
+        std::vector<std::thread> cs;
+        for (int k = 0; k < 2; ++k)
+            cs.emplace_back([&, k] {
+                for (int i = k; i < m; i += 4) cancel_rc[i] = core.cancel(tickets[i]);
+            });
+        for (auto &c : cs) c.join();
+        for (auto &w : ws) w.join();
+        int cancelled = 0;
+        for (int i = 0; i < m; ++i) {
+            ssw_result res;
+            if (i % 4 < 2) {
+                // a waiter may collect the result before the cancel (then
+                // the ticket is gone: NOTFOUND)
+                const bool collected = i % 3 == 0 && wait_rc[i] == SSW_OK;
+                CHECK(cancel_rc[i] == SSW_OK || cancel_rc[i] == SSW_ERR_STATE ||
+                          (collected && cancel_rc[i] == SSW_ERR_NOTFOUND),
+                      "cancel rc %d (waiter %d)", cancel_rc[i], wait_rc[i]);
+                if (cancel_rc[i] == SSW_OK) {
+                    ++cancelled;
+                    // blocked before the cancel: CANCELLED; came after it: NOTFOUND
+                    if (i % 3 == 0)
+                        CHECK(wait_rc[i] == SSW_ERR_CANCELLED || wait_rc[i] == SSW_ERR_NOTFOUND,
+                              "waiter on a cancelled ticket: %d", wait_rc[i]);
+                    CHECK(core.poll(tickets[i], &res) == SSW_ERR_NOTFOUND, "a cancelled ticket is gone");
+                    CHECK(core.cancel(tickets[i]) == SSW_ERR_NOTFOUND, "cancel twice");
+                    continue;
+                }
+                if (collected) continue;
+            }
+            if (i % 3 == 0) {
+                CHECK(wait_rc[i] == SSW_OK, "waiter %d rc %d", i, wait_rc[i]);
+            } else {
+                CHECK(core.wait(tickets[i], 60000, &res) == SSW_OK, "result %d", i);
+                CHECK(res.length == reqs[i].want_len, "result of %d", i);
+            }
+        }
+        CHECK(core.cancel(tickets[0] + 100000) == SSW_ERR_NOTFOUND, "unknown ticket");
+        // what was done stays done
+        CHECK(cancelled > 0 || rep > 0, "no cancel won the race (%d)", cancelled);
+        core.shutdown();
+    }
+}
+
+// A small engine that holds its round until released: a request it has
+// taken cannot be cancelled, one still queued can, and a caller blocked on
+// the queued one returns SSW_ERR_CANCELLED.
+struct GatedSmall final : SmallBackend {
+    std::mutex mu;
+    std::condition_variable cv;
+    bool open = false, entered = false;
+    int solve(size_t m, const uint8_t *n_per, const uint64_t *offs, const uint8_t *delta, int32_t *length,
+              int64_t *expanded, uint8_t *, int, std::string *) override {
+        std::unique_lock<std::mutex> lk(mu);
+        entered = true;
+        cv.notify_all();
+        cv.wait(lk, [&] { return open; });
+        for (size_t i = 0; i < m; ++i) oracle_one(n_per[i], delta + offs[i], &length[i], &expanded[i]);
+        return SSW_OK;
+    }
+};
+
+static void scenario_cancel_blocked_waiter() {
+    GatedSmall gate;
+    StubLarge large;
+    ServerCore core(config(10, 1, false), {&gate}, &large);
+    core.start();
+    std::mt19937 rng(123);
+    Req ra = make_req(rng, 6, 9), rb = make_req(rng, 6, 9);
+    uint64_t ta = 0, tb = 0;
+    CHECK(core.submit(ra.n, ra.a.data(), ra.b.data(), &ta) == SSW_OK, "submit a");
+    {
+        std::unique_lock<std::mutex> lk(gate.mu);
+        gate.cv.wait(lk, [&] { return gate.entered; });  // a's round is in the engine
+    }
+    CHECK(core.submit(rb.n, rb.a.data(), rb.b.data(), &tb) == SSW_OK, "submit b");
+    std::atomic<int> wrc{1};
+    std::thread w([&] {
+        ssw_result res;
+        wrc = core.wait(tb, 60000, &res);
+    });
+    std::this_thread::sleep_for(std::chrono::milliseconds(50));  // the waiter blocks on b
+    CHECK(core.cancel(ta) == SSW_ERR_STATE, "a is being solved");
+    CHECK(core.cancel(tb) == SSW_OK, "b is still queued");
+    w.join();
+    CHECK(wrc.load() == SSW_ERR_CANCELLED, "the blocked waiter: %d", wrc.load());
+    {
+        std::lock_guard<std::mutex> lk(gate.mu);
+        gate.open = true;
+    }
+    gate.cv.notify_all();
+    ssw_result res;
+    CHECK(core.wait(ta, 60000, &res) == SSW_OK && res.length == ra.want_len, "a's result");
+    CHECK(core.poll(tb, &res) == SSW_ERR_NOTFOUND, "b is gone");
+    core.shutdown();
+}
+
 int main() {
+    scenario_cancel_blocked_waiter();
+    scenario_cancel();
     scenario_many_callers();
     scenario_two_waiters_one_ticket();
     scenario_timeout_and_budget();

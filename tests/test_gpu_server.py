@@ -116,3 +116,37 @@ def test_destroy_with_requests_in_flight(ctx):
         a, b = ssw.generate_cerny(9, 1)
         assert srv2.wait(srv2.submit(9, a, b))["length"] == 64
         assert srv2.stats()["completed"] == 1
+
+
+def test_cancel_queued_requests(ctx):
+    # one large instance at a time: behind a long C_26 the other large
+    # requests wait in the queue, so cancels of them win (their tickets are
+    # then gone); the running one cannot be cancelled and its result arrives;
+    # what is not cancelled equals the oracle
+    rng = np.random.default_rng(23)
+    with ssw.Server(ctx, small_n_max=16, max_large=1) as srv:
+        a, b = ssw.generate_cerny(26, 0)
+        first = srv.submit(26, a, b)
+        reqs, tickets = [], []
+        for i in range(20):
+            n = 21 + i % 2
+            a = rng.integers(0, n, size=n).astype(np.int32)
+            b = rng.integers(0, n, size=n).astype(np.int32)
+            reqs.append((n, a, b))
+            tickets.append(srv.submit(n, a, b))
+        time.sleep(0.002)  # the engine has taken the first request
+        cancelled = {i for i in range(0, 20, 2) if srv.cancel(tickets[i])}
+        assert len(cancelled) >= 8, cancelled
+        assert srv.cancel(first) is False
+        assert srv.wait(first)["length"] == 25 * 25
+        for i in range(20):
+            if i in cancelled:
+                with pytest.raises(ssw.SswError) as e:
+                    srv.poll(tickets[i])
+                assert e.value.code == ssw.SSW_ERR_NOTFOUND
+                continue
+            n, a, b = reqs[i]
+            r = srv.wait(tickets[i])
+            L, S, E, _ = O.solve(n, a[None, :], b[None, :])
+            assert r["length"] == L[0] and r["expanded"] == E[0], (i, n, r)
+        assert srv.stats()["completed"] == 21 - len(cancelled)
