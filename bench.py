@@ -227,14 +227,20 @@ def bench_uneven(ctx):
 def bench_large(ctx):
     items = workloads.large()
     ns = [it[0] for it in items]
+    # the first call allocates the instances' bitmaps and rings; the engine
+    # keeps them for the next call (a resident engine's steady state)
+    t0 = time.perf_counter()
+    ctx.solve_mixed(ns, [it[1] for it in items], [it[2] for it in items], mode="large")
+    first_dt = time.perf_counter() - t0
     t0 = time.perf_counter()
     r = ctx.solve_mixed(ns, [it[1] for it in items], [it[2] for it in items], mode="large")
     dt = time.perf_counter() - t0
     cerny_ok = all(int(r.length[i]) == (n - 1) ** 2 for i, (n, a, b, k) in enumerate(items) if k == "cerny")
     hbm_ceiling = ctx.measure_atomics("hbm")
     tas = 2.0 * float(r.expanded.sum()) / dt
-    return {"workload": "Cerny C_24..C_30 + 64 random n=30 (seed 1004), all concurrent, device-wide path",
-            "wall_s": dt, "subsets": int(r.expanded.sum()), "subsets_per_s": float(r.expanded.sum()) / dt,
+    return {"workload": "Cerny C_24..C_30 + 64 random n=30 (seed 1004), all concurrent, device-wide path; the second of two calls (the first allocates the instances' buffers, first_call_wall_s)",
+            "wall_s": dt, "first_call_wall_s": first_dt, "subsets": int(r.expanded.sum()),
+            "subsets_per_s": float(r.expanded.sum()) / dt,
             "atomics": {"where": "HBM", "achieved_per_s": tas, "ceiling_per_s": hbm_ceiling,
                         "frac": tas / hbm_ceiling,
                         "note": "two test-and-sets per expanded subset over the wall time (host loop included); "

@@ -136,6 +136,16 @@ struct LargeEngine {
     int tick_levels = 32;
     uint64_t bytes_in_use = 0;
     std::map<int, hipGraphExec_t> graphs;  // captured [step, plan] chains by level count
+    // Device buffers of retired instances (bitmaps, rings, tables, level
+    // tables), kept by size for the next admissions: allocating costs about
+    // 0.4 ms per 128 MiB, a quarter of the C_24..C_30 set's wall time when
+    // every admission allocated.  At most spare_limit bytes; freed with the
+    // engine, or when an allocation fails.
+    std::multimap<uint64_t, void *> spare;
+    uint64_t spare_bytes = 0, spare_limit = 0;
+    hipError_t take(void **p, uint64_t bytes);
+    void give(void *p, uint64_t bytes);
+    void drop_spare();
 
     LargeState *dstate() const;
     LargeInstance *dinst() const;

@@ -50,15 +50,51 @@ LargeInstance *LargeEngine::dinst() const {
 void LargeEngine::release() {
     for (auto &s : slots) free_slot_buffers(s);
     slots.clear();
+    drop_spare();
     for (auto &kv : graphs) hipGraphExecDestroy(kv.second);
     graphs.clear();
     if (state.p) hipFree(state.p);
     state = DevBuf();
 }
 
+hipError_t LargeEngine::take(void **p, uint64_t bytes) {
+    auto it = spare.find(bytes);
+    if (it != spare.end()) {
+        *p = it->second;
+        spare.erase(it);
+        spare_bytes -= bytes;
+        return hipSuccess;
+    }
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess && !spare.empty()) {
+        (void)hipGetLastError();
+        drop_spare();
+        e = hipMalloc(p, bytes);
+    }
+    return e;
+}
+
+void LargeEngine::give(void *p, uint64_t bytes) {
+    if (!p) return;
+    if (spare_bytes + bytes <= spare_limit) {
+        spare.emplace(bytes, p);
+        spare_bytes += bytes;
+    } else {
+        hipFree(p);
+    }
+}
+
+void LargeEngine::drop_spare() {
+    for (auto &kv : spare) hipFree(kv.second);
+    spare.clear();
+    spare_bytes = 0;
+}
+
 void LargeEngine::free_slot_buffers(LargeSlot &s) {
-    for (void *p : {(void *)s.ring, (void *)s.bitmap, (void *)s.table, (void *)s.lvl})
-        if (p) hipFree(p);
+    give(s.ring, 4ull * s.ring_cap);
+    give(s.bitmap, std::max<uint64_t>(256, (1ull << s.n) / 8));
+    give(s.table, 8192);
+    give(s.lvl, 8ull * s.lvl_cap);
     s.ring = nullptr;
     s.bitmap = nullptr;
     s.table = nullptr;
@@ -80,6 +116,7 @@ int LargeEngine::setup(ssw_context &ctx) {
         int per_cu = 1;
         if (const char *e = getenv("SSW_LARGE_GRID_PER_CU")) per_cu = std::max(1, std::min(32, atoi(e)));
         grid = std::max(1, ctx.cus) * per_cu;
+        spare_limit = ctx.budget / 4;
         if (const char *e = getenv("SSW_LARGE_LEVELS")) tick_levels = std::max(1, std::min(256, atoi(e)));
     }
     return SSW_OK;
@@ -138,17 +175,17 @@ int LargeEngine::admit(ssw_context &ctx, int64_t index, int n, const uint8_t *de
     LargeSlot &S = slots[slot];
     S = LargeSlot();
     const uint64_t bm_bytes = std::max<uint64_t>(256, all / 8);
-    LG_TRY(hipMalloc((void **)&S.bitmap, bm_bytes), "bitmap");
-    LG_TRY(hipMalloc((void **)&S.ring, 4 * ring_cap), "ring");
-    LG_TRY(hipMalloc((void **)&S.table, 8192), "table");
+    S.n = n;
+    S.ring_cap = ring_cap;
     S.lvl_cap = append ? (uint32_t)((n - 1) * (n - 1) + 8) : 0;
-    if (append) LG_TRY(hipMalloc((void **)&S.lvl, 8ull * S.lvl_cap), "levels");
+    LG_TRY(take((void **)&S.bitmap, bm_bytes), "bitmap");
+    LG_TRY(take((void **)&S.ring, 4 * ring_cap), "ring");
+    LG_TRY(take((void **)&S.table, 8192), "table");
+    if (append) LG_TRY(take((void **)&S.lvl, 8ull * S.lvl_cap), "levels");
     S.bytes = need;
     bytes_in_use += need;
     S.used = true;
     S.index = index;
-    S.n = n;
-    S.ring_cap = ring_cap;
     S.append = append;
     S.harvested = false;
     LG_TRY(fill_u32(S.bitmap, bm_bytes / 4, 0, ctx.stream), "clear bitmap");
@@ -237,11 +274,11 @@ int LargeEngine::read_back(ssw_context &ctx) {
                 return ctx.fail(SSW_ERR_INTERNAL, "device-wide path: ring of %llu entries for a level needing %llu",
                                 (unsigned long long)cap, (unsigned long long)need);
             uint32_t *nr = nullptr;
-            LG_TRY(hipMalloc((void **)&nr, 4 * cap), "grow ring");
+            LG_TRY(take((void **)&nr, 4 * cap), "grow ring");
             const uint64_t from = I.append ? 0 : I.lo;
             LG_TRY(large_launch_regrow(S.ring, S.ring_cap - 1, nr, cap - 1, from, I.hi, ctx.stream), "regrow");
             LG_TRY(hipStreamSynchronize(ctx.stream), "sync");
-            hipFree(S.ring);
+            give(S.ring, 4 * S.ring_cap);
             bytes_in_use += 4 * (cap - S.ring_cap);
             S.bytes += 4 * (cap - S.ring_cap);
             S.ring = nr;
@@ -253,10 +290,10 @@ int LargeEngine::read_back(ssw_context &ctx) {
             // more levels than (n-1)^2 + 8: a Cerny-bound violation; keep going
             const uint32_t cap = I.lvl_cap * 2;
             uint64_t *nl = nullptr;
-            LG_TRY(hipMalloc((void **)&nl, 8ull * cap), "grow levels");
+            LG_TRY(take((void **)&nl, 8ull * cap), "grow levels");
             LG_TRY(hipMemcpyAsync(nl, S.lvl, 8ull * I.lvl_cap, hipMemcpyDeviceToDevice, ctx.stream), "copy levels");
             LG_TRY(hipStreamSynchronize(ctx.stream), "sync");
-            hipFree(S.lvl);
+            give(S.lvl, 8ull * S.lvl_cap);
             S.lvl = nl;
             S.lvl_cap = cap;
             I.lvl = nl;
@@ -283,7 +320,7 @@ int LargeEngine::witness(ssw_context &ctx, int slot, uint8_t *out, int stride) {
     const int L = I.length;
     const uint64_t words = std::max<uint64_t>(1, (1ull << S.n) / 32);
     uint32_t *good = nullptr;
-    LG_TRY(hipMalloc((void **)&good, 8 * words), "witness bitmaps");
+    LG_TRY(take((void **)&good, 8 * words), "witness bitmaps");
     LG_TRY(fill_u32(good, 2 * words, 0, ctx.stream), "clear");
     uint32_t *g0 = good, *g1 = good + words;
     for (int j = L - 1; j >= 0; --j) {
@@ -293,7 +330,7 @@ int LargeEngine::witness(ssw_context &ctx, int slot, uint8_t *out, int stride) {
     }
     LG_TRY(large_launch_fwd(dinst(), slot, L, g0, g1, out, stride, ctx.stream), "large_fwd");
     LG_TRY(hipStreamSynchronize(ctx.stream), "sync");
-    hipFree(good);
+    give(good, 8 * words);
     ctx.total_launches += L + 1;
     return SSW_OK;
 }
