@@ -25,6 +25,7 @@
 //     shortest path to a singleton, a forward pass takes letter a whenever
 //     its image is such a subset of the next level.
 #include "ssw_internal.h"
+#include "ssw_checked.h"
 
 #include <mutex>
 
@@ -531,13 +532,13 @@ struct GroupSolver {
     }
 
     __device__ __forceinline__ uint32_t aget(uint32_t p) const {
-        return p < P.lds_arena_cap ? arena[p] : garena[p - P.lds_arena_cap];
+        return p < P.lds_arena_cap ? arena[p] : garena[SSW_IDX(p - P.lds_arena_cap, P.garena_cap)];
     }
     __device__ __forceinline__ void aput(uint32_t p, uint32_t v) const {
         if (p < P.lds_arena_cap)
             arena[p] = v;
         else
-            garena[p - P.lds_arena_cap] = v;
+            garena[SSW_IDX(p - P.lds_arena_cap, P.garena_cap)] = v;
     }
     __device__ __forceinline__ uint64_t image(uint32_t s) const {
         if constexpr (FAM) {
@@ -567,9 +568,9 @@ struct GroupSolver {
         const uint32_t bit = 1u << (x & 31u);
         uint32_t old;
         if (lds_bm)
-            old = or_wg(bm + (x >> 5), bit);
+            old = or_wg(bm + SSW_IDX(x >> 5, P.lds_bitmap_words), bit);
         else
-            old = or_wg(gbm + (x >> 5), bit);
+            old = or_wg(gbm + SSW_IDX(x >> 5, P.gbitmap_words), bit);
         return (old & bit) == 0u;
     }
 
@@ -850,14 +851,14 @@ struct GroupSolver {
             lo = hi;
             hi = wr;
             rd = lo;
-            if (P.want_witness && g.rank() == 0) lvl[level + 1] = hi;
+            if (P.want_witness && g.rank() == 0) lvl[SSW_IDX(level + 1, P.lvl_cap)] = hi;
             if (lo == hi) return kNoSync;
         }
         return kContinue;
     }
 
-    __device__ __forceinline__ static bool bit_of(const glb_u32 *m, uint32_t x) {
-        return (m[x >> 5] >> (x & 31u)) & 1u;
+    __device__ __forceinline__ bool bit_of(const glb_u32 *m, uint32_t x) const {
+        return (m[SSW_IDX(x >> 5, P.good_words)] >> (x & 31u)) & 1u;
     }
 
     // Lexicographically smallest shortest reset word from the retained levels.
@@ -880,7 +881,7 @@ struct GroupSolver {
                     const uint32_t a = (uint32_t)im, b = (uint32_t)(im >> 32);
                     const bool on = (j == L - 1) ? (is_single(a) || is_single(b))
                                                  : (bit_of(src, a) || bit_of(src, b));
-                    if (on) or_wg(dst + (x >> 5), 1u << (x & 31u));
+                    if (on) or_wg(dst + SSW_IDX(x >> 5, P.good_words), 1u << (x & 31u));
                 }
             }
             g.sync();
@@ -918,8 +919,8 @@ struct GroupSolver {
         const uint32_t end = lvl[L];
         for (uint32_t p = r; p < end; p += G) {
             const uint32_t x = aget(p);
-            g0[x >> 5] = 0;
-            g1[x >> 5] = 0;
+            g0[SSW_IDX(x >> 5, P.good_words)] = 0;
+            g1[SSW_IDX(x >> 5, P.good_words)] = 0;
         }
         g.sync();
     }
@@ -951,10 +952,10 @@ struct GroupSolver {
         }
         if (!lds_bm) {
             g.sync();
-            for (uint32_t p = r; p < wr; p += G) gbm[aget(p) >> 5] = 0;
+            for (uint32_t p = r; p < wr; p += G) gbm[SSW_IDX(aget(p) >> 5, P.gbitmap_words)] = 0;
         } else if (FAM && fam) {
             g.sync();
-            for (uint32_t p = r; p < wr; p += G) bm[aget(p) >> 5] = 0;
+            for (uint32_t p = r; p < wr; p += G) bm[SSW_IDX(aget(p) >> 5, P.lds_bitmap_words)] = 0;
         }
         g.sync();
         if (done_local >= 256) flush();
@@ -1040,6 +1041,8 @@ struct SmallSolver {
     glb_u16 *hist;   // WIT: subsets in discovery order, 65536 entries and 64 dummies
     glb_u8 *goodp;   // WIT: 1 = the subset at this position of hist is on a shortest path
     glb_u32 *lvlp;   // WIT: start position of each level in hist
+    static constexpr uint32_t kHistEntries = 65536u + 64u;
+    static constexpr uint32_t kLevelEntries = 65536u + 4u;
     glb_u32 *choicep;  // WIT: bit x = letter a keeps good subset x on a shortest path
     uint32_t rmask, rcap;
     bool in_lds;
@@ -1096,10 +1099,32 @@ struct SmallSolver {
         found_end = 0;
         nx_ai = -1;
     }
+#ifdef SSW_CHECKED
+    // Checked build: every LDS address the solver forms lies in its own
+    // visited bitmap or in its own region (tables, staging, singleton images,
+    // control words, ring); one that does not is reported with the line that
+    // formed it and goes to the staging words instead.
+    template <class V>
+    __device__ __forceinline__ V *lds_at(uint32_t a, uint32_t line) const {
+        const uint32_t w = (uint32_t)sizeof(V);
+        const bool in_bm = a >= bm_a && a + w <= bm_a + 4u * P.lds_bitmap_words;
+        const bool in_region = a >= tab_a && a + w <= tab_a + P.lds_group_bytes;
+        if (!in_bm && !in_region) {
+            checked_report(line, a);
+            a = stage_a;
+        }
+        return (V *)(size_t)a;
+    }
+#define l32(a) lds_at<lds_u32>((a), __LINE__)
+#define l16(a) lds_at<lds_u16>((a), __LINE__)
+#define l8(a) lds_at<lds_u8>((a), __LINE__)
+#define l128(a) lds_at<lds_u4>((a), __LINE__)
+#else
     static __device__ __forceinline__ lds_u32 *l32(uint32_t a) { return (lds_u32 *)(size_t)a; }
     static __device__ __forceinline__ lds_u16 *l16(uint32_t a) { return (lds_u16 *)(size_t)a; }
     static __device__ __forceinline__ lds_u8 *l8(uint32_t a) { return (lds_u8 *)(size_t)a; }
     static __device__ __forceinline__ lds_u4 *l128(uint32_t a) { return (lds_u4 *)(size_t)a; }
+#endif
 
     // LDS-only ordering inside one wavefront needs no memory fence: LDS
     // instructions of a wave complete in order
@@ -1665,7 +1690,7 @@ struct SmallSolver {
             // (one per lane) instead of branching around the store
 #pragma unroll
             for (int j = 0; j < 2 * TT; ++j)
-                hist[nw[j] ? (pos[j] & 0xFFFFu) : 65536u + (r & 63u)] =
+                hist[SSW_IDX(nw[j] ? (pos[j] & 0xFFFFu) : 65536u + (r & 63u), kHistEntries)] =
                     (uint16_t)((j & 1) ? (img[j >> 1] >> 16) : (img[j >> 1] & 0xFFFFu));
         }
 #pragma unroll
@@ -1705,7 +1730,7 @@ struct SmallSolver {
 
     // ---- witness words (WIT) ----
     __device__ void record_level() {
-        if (g.rank() == 0) lvlp[level + 1u] = hi;
+        if (g.rank() == 0) lvlp[SSW_IDX(level + 1u, kLevelEntries)] = hi;
     }
     __device__ glb_u8 *row() const { return (glb_u8 *)(P.witness + (uint64_t)ai * (uint64_t)P.witness_stride); }
     // zero bytes from..stride-1 of the automaton's witness row
@@ -1747,19 +1772,19 @@ struct SmallSolver {
         return w;
     }
     __device__ __forceinline__ uint32_t level_start(const WitnessSpace &w, uint32_t k) const {
-        return w.starts_cached ? uni<G>(*l32(w.starts_a + 4u * k)) : uni<G>(lvlp[k]);
+        return w.starts_cached ? uni<G>(*l32(w.starts_a + 4u * k)) : uni<G>(lvlp[SSW_IDX(k, kLevelEntries)]);
     }
     __device__ __forceinline__ uint32_t hist_at(const WitnessSpace &w, uint32_t p) const {
-        return w.local ? (uint32_t)*l16(ring_a + 2u * p) : (uint32_t)hist[p];
+        return w.local ? (uint32_t)*l16(ring_a + 2u * p) : (uint32_t)hist[SSW_IDX(p, kHistEntries)];
     }
     __device__ __forceinline__ uint32_t flag_at(const WitnessSpace &w, uint32_t p) const {
-        return w.local ? (uint32_t)*l8(w.flags_a + p) : (uint32_t)goodp[p];
+        return w.local ? (uint32_t)*l8(w.flags_a + p) : (uint32_t)goodp[SSW_IDX(p, 65536u)];
     }
     __device__ __forceinline__ void set_flag(const WitnessSpace &w, uint32_t p, bool on) const {
         if (w.local)
             *l8(w.flags_a + p) = on ? 1u : 0u;
         else
-            goodp[p] = on ? 1u : 0u;
+            goodp[SSW_IDX(p, 65536u)] = on ? 1u : 0u;
     }
     // The lexicographically smallest shortest reset word (a before b), the
     // word the reference's back-tracking yields (CPU.cs:262-269).  Backward,
@@ -1780,7 +1805,7 @@ struct SmallSolver {
         wave_sync();  // the recorded subsets are visible to every lane
         clear_bitmap();
         if (w.starts_cached)
-            for (uint32_t k = r; k <= (uint32_t)L; k += G) *l32(w.starts_a + 4u * k) = lvlp[k];
+            for (uint32_t k = r; k <= (uint32_t)L; k += G) *l32(w.starts_a + 4u * k) = lvlp[SSW_IDX(k, kLevelEntries)];
         lsync();
         if (!w.local) {
             witness_hbm(L, w);
@@ -1896,7 +1921,7 @@ struct SmallSolver {
 #pragma unroll
             for (uint32_t u = 0; u < U; ++u) {
                 const uint32_t p = base + u * G + r;
-                x[u] = p < e ? (uint32_t)hist[p] : 0u;
+                x[u] = p < e ? (uint32_t)hist[SSW_IDX(p, kHistEntries)] : 0u;
             }
 #pragma unroll
             for (uint32_t u = 0; u < U; ++u) {
@@ -1910,7 +1935,7 @@ struct SmallSolver {
                     if (in_lds)
                         *l16(keep_a + 2u * (p - s)) = (uint16_t)(on ? x[u] : 0u);  // no subset is empty
                     else
-                        goodp[p] = on ? 1u : 0u;
+                        goodp[SSW_IDX(p, 65536u)] = on ? 1u : 0u;
                 }
             }
         }
@@ -1926,7 +1951,7 @@ struct SmallSolver {
                 if (in_lds)
                     x[u] = p < e ? (uint32_t)*l16(keep_a + 2u * (p - s)) : 0u;
                 else
-                    x[u] = (p < e && goodp[p]) ? (uint32_t)hist[p] : 0u;
+                    x[u] = (p < e && goodp[SSW_IDX(p, 65536u)]) ? (uint32_t)hist[SSW_IDX(p, kHistEntries)] : 0u;
             }
 #pragma unroll
             for (uint32_t u = 0; u < U; ++u)
@@ -1936,7 +1961,8 @@ struct SmallSolver {
     }
     // bit x of the choice bitmap: letter a keeps good subset x on a shortest path
     __device__ __forceinline__ void choose(uint32_t x) const {
-        __hip_atomic_fetch_or(choicep + (x >> 5), 1u << (x & 31u), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_or(choicep + SSW_IDX(x >> 5, P.lds_bitmap_words), 1u << (x & 31u), __ATOMIC_RELAXED,
+                              __HIP_MEMORY_SCOPE_WORKGROUP);
     }
 
     // The passes for a history in HBM with the good subsets as lists in LDS.
@@ -1968,7 +1994,7 @@ struct SmallSolver {
 #pragma unroll
                     for (uint32_t u = 0; u < U; ++u) {
                         const uint32_t p = base + u * G + r;
-                        x[u] = p < e ? (uint32_t)hist[p] : 0u;
+                        x[u] = p < e ? (uint32_t)hist[SSW_IDX(p, kHistEntries)] : 0u;
                     }
                 }
 #pragma unroll
@@ -1994,7 +2020,7 @@ struct SmallSolver {
 #pragma unroll
                 for (uint32_t u = 0; u < U; ++u) {
                     const uint32_t p = s2 + u * G + r;
-                    x[u] = p < s ? (uint32_t)hist[p] : 0u;
+                    x[u] = p < s ? (uint32_t)hist[SSW_IDX(p, kHistEntries)] : 0u;
                 }
                 have = true;
             }
@@ -2070,8 +2096,9 @@ struct SmallSolver {
         const uint32_t words = P.lds_bitmap_words;
         for (uint32_t k = r; k < words; k += G) {
             // read where the atomics were performed (past the vector L1)
-            *l32(bm_a + 4u * k) = __hip_atomic_load(choicep + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            choicep[k] = 0u;  // clean for the next automaton
+            *l32(bm_a + 4u * k) = __hip_atomic_load(choicep + SSW_IDX(k, P.lds_bitmap_words), __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+            choicep[SSW_IDX(k, P.lds_bitmap_words)] = 0u;  // clean for the next automaton
         }
         lsync();
         glb_u8 *out = row();
@@ -2144,6 +2171,12 @@ struct SmallSolver {
         expanded_local = 0;
     }
 };
+#ifdef SSW_CHECKED
+#undef l32
+#undef l16
+#undef l8
+#undef l128
+#endif
 
 template <class Solver, class Grp>
 __device__ __forceinline__ void batch_body(const BatchParams &P) {
@@ -2378,5 +2411,9 @@ hipError_t fill_u32(uint32_t *p, uint64_t words, uint32_t value, hipStream_t st)
     hipLaunchKernelGGL(fill_u32_kernel, dim3((unsigned)blocks), dim3(256), 0, st, p, words, value);
     return hipGetLastError();
 }
+
+#ifdef SSW_CHECKED
+SSW_CHECKED_TAKE(checked_take_batch)
+#endif
 
 }  // namespace ssw

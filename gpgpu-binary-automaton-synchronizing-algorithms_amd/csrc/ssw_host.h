@@ -270,6 +270,9 @@ struct ssw_context {
     // witness; every device buffer it uses is the context's (set up at
     // creation) or this GPU's static scratch
     int solve_noalloc(const ssw_batch *b, int64_t *violation_out);
+    // checked build: SSW_ERR_INTERNAL if a kernel formed an index or LDS
+    // address outside its buffer since the last call (SSW_OK otherwise)
+    int checked_verdict();
     void *noalloc_base = nullptr;       // device address of this GPU's scratch
     hipError_t noalloc_status = hipSuccess;
     int solve_run(const ssw_batch *b, int32_t memory, int mode, int64_t *violation_out);

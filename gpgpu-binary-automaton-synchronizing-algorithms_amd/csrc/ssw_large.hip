@@ -22,10 +22,14 @@
 //     the next level; large_fwd then reads off the lexicographically smallest
 //     shortest word.
 #include "ssw_internal.h"
+#include "ssw_checked.h"
 
 namespace ssw {
 
 __device__ __forceinline__ bool lg_single(uint32_t x) { return (x & (x - 1u)) == 0u; }
+
+// u32 words of a bitmap over the 2^n subsets of an n-state automaton
+__device__ __forceinline__ uint64_t lg_words(int n) { return n > 5 ? 1ull << (n - 5) : 1ull; }
 
 __device__ __forceinline__ uint32_t or_agent(uint32_t *p, uint32_t v) {
     return __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -215,11 +219,11 @@ __global__ __launch_bounds__(256) void large_step(LargeState *st, LargeInstance 
             bool na = false, nb = false;
             if (valid && !sa) {
                 const uint32_t bit = 1u << (a & 31u);
-                na = (or_agent(bitmap + (a >> 5), bit) & bit) == 0u;
+                na = (or_agent(bitmap + SSW_IDX(a >> 5, lg_words(I.n)), bit) & bit) == 0u;
             }
             if (valid && !sb) {
                 const uint32_t bit = 1u << (b & 31u);
-                nb = (or_agent(bitmap + (b >> 5), bit) & bit) == 0u;
+                nb = (or_agent(bitmap + SSW_IDX(b >> 5, lg_words(I.n)), bit) & bit) == 0u;
             }
             const uint64_t ba = __ballot(na), bb = __ballot(nb);
             const uint32_t wtot = __popcll(ba) + __popcll(bb);
@@ -227,8 +231,8 @@ __global__ __launch_bounds__(256) void large_step(LargeState *st, LargeInstance 
             if (lane == 0 && wtot) wbase = atomicAdd(&s_count, wtot);
             wbase = __shfl(wbase, 0, 64);
             uint32_t k = wbase + __popcll(ba & below) + __popcll(bb & below);
-            if (na) outbuf[k++] = a;
-            if (nb) outbuf[k] = b;
+            if (na) outbuf[SSW_IDX(k++, 2u * kLargeTile)] = a;
+            if (nb) outbuf[SSW_IDX(k, 2u * kLargeTile)] = b;
         }
         if (__ballot(single) != 0 && lane == 0) atomicOr(&I.found, 1);
         __syncthreads();
@@ -270,15 +274,17 @@ __global__ __launch_bounds__(256) void large_back(const LargeInstance *inst, int
     const LargeInstance &I = inst[slot];
     for (int e = threadIdx.x; e < 1024; e += blockDim.x) tab[e] = I.table[e];
     __syncthreads();
-    const uint64_t s = I.lvl[j], e = I.lvl[j + 1];
+    const uint64_t s = I.lvl[SSW_IDX(j, I.lvl_cap)], e = I.lvl[SSW_IDX(j + 1, I.lvl_cap)];
     for (uint64_t p = s + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < e;
          p += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t x = I.ring[p & I.ring_mask];
         const uint64_t im = lg_image(tab, x);
         const uint32_t a = (uint32_t)im, b = (uint32_t)(im >> 32);
+        const uint64_t words = lg_words(I.n);
         const bool on = (j == L - 1) ? (lg_single(a) || lg_single(b))
-                                     : (((src[a >> 5] >> (a & 31u)) & 1u) || ((src[b >> 5] >> (b & 31u)) & 1u));
-        if (on) or_agent(dst + (x >> 5), 1u << (x & 31u));
+                                     : (((src[SSW_IDX(a >> 5, words)] >> (a & 31u)) & 1u) ||
+                                        ((src[SSW_IDX(b >> 5, words)] >> (b & 31u)) & 1u));
+        if (on) or_agent(dst + SSW_IDX(x >> 5, words), 1u << (x & 31u));
     }
 }
 
@@ -301,10 +307,10 @@ __global__ __launch_bounds__(256) void large_fwd(const LargeInstance *inst, int 
         } else {
             const uint32_t *gs = ((k + 1) & 1) ? g1 : g0;
             take_a = false;
-            if ((gs[a >> 5] >> (a & 31u)) & 1u) {
+            if ((gs[SSW_IDX(a >> 5, lg_words(I.n))] >> (a & 31u)) & 1u) {
                 if (threadIdx.x == 0) hit = 0;
                 __syncthreads();
-                const uint64_t s = I.lvl[k + 1], e = I.lvl[k + 2];
+                const uint64_t s = I.lvl[SSW_IDX(k + 1, I.lvl_cap)], e = I.lvl[SSW_IDX(k + 2, I.lvl_cap)];
                 for (uint64_t base = s; base < e; base += blockDim.x) {
                     const uint64_t p = base + threadIdx.x;
                     if (p < e && I.ring[p & I.ring_mask] == a) hit = 1;
@@ -366,5 +372,9 @@ hipError_t large_launch_fwd(const LargeInstance *inst, int slot, int L, const ui
     hipLaunchKernelGGL(large_fwd, dim3(1), dim3(256), 0, st, inst, slot, L, g0, g1, out, stride);
     return hipGetLastError();
 }
+
+#ifdef SSW_CHECKED
+SSW_CHECKED_TAKE(checked_take_large)
+#endif
 
 }  // namespace ssw

@@ -31,6 +31,10 @@ size_t batch_group_lds(int variant, uint32_t bitmap_words, uint32_t arena_cap);
 size_t batch_block_lds(int variant, int group_lanes, int groups, uint32_t bitmap_words, uint32_t arena_cap);
 hipError_t fill_u32(uint32_t *p, uint64_t words, uint32_t value, hipStream_t st);
 hipError_t count_sizes(const uint8_t *n_per, int64_t count, unsigned long long *hist, hipStream_t st);
+#ifdef SSW_CHECKED
+hipError_t checked_take_batch(hipStream_t s, unsigned long long *out);
+hipError_t checked_take_large(hipStream_t s, unsigned long long *out);
+#endif
 hipError_t init_queues(uint32_t *queues, int64_t count, hipStream_t st, bool empty);
 hipError_t init_queues3(uint32_t *q0, uint32_t *q1, uint32_t *q2, int64_t count, uint32_t empty_mask,
                         hipStream_t st);
@@ -1003,9 +1007,25 @@ int ssw_context::solve_entry(const ssw_batch *b, int32_t memory, int mode, int64
     if (b->count == 0) return SSW_OK;
     HIP_TRY(hipSetDevice(device), "hipSetDevice");
     solve_begin((uint64_t)b->count);
-    const int rc = solve_run(b, memory, mode, violation_out);
+    int rc = solve_run(b, memory, mode, violation_out);
     solve_end();
-    return rc;
+    const int crc = checked_verdict();
+    return rc ? rc : crc;
+}
+
+int ssw_context::checked_verdict() {
+#ifdef SSW_CHECKED
+    // bit 63: set; bits 40..62: source line; bits 0..39: the index or address
+    unsigned long long v[2] = {0, 0};
+    HIP_TRY(checked_take_batch(stream, &v[0]), "reading the bounds checks (ssw_batch.hip)");
+    HIP_TRY(checked_take_large(stream, &v[1]), "reading the bounds checks (ssw_large.hip)");
+    for (int k = 0; k < 2; ++k)
+        if (v[k])
+            return fail(SSW_ERR_INTERNAL, "bounds check failed: %s line %u formed index or LDS address %llu",
+                        k == 0 ? "ssw_batch.hip" : "ssw_large.hip", (unsigned)((v[k] >> 40) & 0x7FFFFFu),
+                        v[k] & ((1ull << 40) - 1));
+#endif
+    return SSW_OK;
 }
 
 int ssw_context::solve_run(const ssw_batch *b, int32_t memory, int mode, int64_t *violation_out) {
@@ -1461,7 +1481,9 @@ int ssw_solve_batch(ssw_context *ctx, const ssw_batch *b, int32_t memory, int64_
 
 int ssw_solve_batch_noalloc(ssw_context *ctx, const ssw_batch *b, int64_t *violation_out) {
     CTX_GUARD(ctx);
-    return ctx->solve_noalloc(b, violation_out);
+    const int rc = ctx->solve_noalloc(b, violation_out);
+    const int crc = ctx->checked_verdict();
+    return rc ? rc : crc;
 }
 
 int ssw_solve_plain(ssw_context *ctx, const ssw_batch *b, int32_t memory, int64_t *violation_out) {

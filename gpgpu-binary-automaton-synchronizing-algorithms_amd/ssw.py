@@ -109,16 +109,19 @@ class LaunchStat(ctypes.Structure):
 
 PROGRESS_FN = ctypes.CFUNCTYPE(None, ctypes.POINTER(Progress), ctypes.c_void_p)
 
-_lib = None
+CHECKED_LIB_PATH = os.path.join(HERE, "libssw_checked.so")
+
+_libs = {}
 
 
-def lib():
-    """Loads libssw.so; raises when it has not been built."""
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
-        L = ctypes.CDLL(LIB_PATH)
+def lib(path=None):
+    """Loads libssw.so (or another build of it, e.g. the bounds-checked
+    libssw_checked.so); raises when it has not been built."""
+    path = path or LIB_PATH
+    if path not in _libs:
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is missing: run __graft_entry__.build() first")
+        L = ctypes.CDLL(path)
         vp, i32, i64, u64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64
         sig = {
             "ssw_abi_version": ([], ctypes.c_int),
@@ -165,8 +168,8 @@ def lib():
             f = getattr(L, name)
             f.argtypes = args
             f.restype = res
-        _lib = L
-    return _lib
+        _libs[path] = L
+    return _libs[path]
 
 
 def _ptr(a):
@@ -256,8 +259,8 @@ class Solved:
 class Context:
     """One solver context on one HIP device (a stream, resident scratch)."""
 
-    def __init__(self, device: int = 0, budget: int = 0):
-        self._L = lib()
+    def __init__(self, device: int = 0, budget: int = 0, library: str = None):
+        self._L = lib(library)
         h = ctypes.c_void_p()
         rc = self._L.ssw_create(device, budget, 0, ctypes.byref(h))
         if rc:

@@ -1,6 +1,7 @@
 """The C-ABI library and the host-side plumbing -- CPU only (no compute calls).
 
-* libssw.so loads and exports every function declared in include/ssw.h;
+* libssw.so and its bounds-checked build libssw_checked.so load and export
+  every function declared in include/ssw.h;
 * the ctypes structures match the C layout (sizeof/offsetof from gcc);
 * generators: the counter-based uniform generator against a numpy twin,
   the two Cerny conventions (Problem.cs:13-29), the System.Random twin;
@@ -26,8 +27,11 @@ def declared_functions():
     return sorted(set(re.findall(r"^(?:int|const char \*)\s*(ssw_\w+)\s*\(", text, re.M)))
 
 
-def test_library_exports_every_declared_symbol():
-    lib = ctypes.CDLL(ssw.LIB_PATH)
+@pytest.mark.parametrize("path", [ssw.LIB_PATH, ssw.CHECKED_LIB_PATH], ids=["libssw", "libssw_checked"])
+def test_library_exports_every_declared_symbol(path):
+    if path == ssw.CHECKED_LIB_PATH and not os.path.exists(path):
+        pytest.skip("libssw_checked.so is not built (build.py --checked)")
+    lib = ctypes.CDLL(path)
     names = declared_functions()
     assert len(names) >= 30
     missing = [n for n in names if not hasattr(lib, n)]
