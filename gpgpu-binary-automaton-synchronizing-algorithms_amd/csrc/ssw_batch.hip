@@ -43,6 +43,7 @@ typedef __attribute__((address_space(3))) uint16_t lds_u16;
 typedef __attribute__((address_space(3))) uint8_t lds_u8;
 typedef __attribute__((address_space(1))) uint32_t glb_u32;
 typedef __attribute__((address_space(1))) uint16_t glb_u16;
+typedef __attribute__((address_space(1))) uint64_t glb_u64;
 typedef __attribute__((address_space(1))) uint8_t glb_u8;
 
 template <class T>
@@ -1039,6 +1040,7 @@ struct SmallSolver {
     uint32_t dum_a;
     glb_u16 *gring;
     glb_u16 *hist;   // WIT: subsets in discovery order, 65536 entries and 64 dummies
+    glb_u32 *himg;   // WIT, T >= 4: the images of the subset at each history position (once expanded)
     glb_u8 *goodp;   // WIT: 1 = the subset at this position of hist is on a shortest path
     glb_u32 *lvlp;   // WIT: start position of each level in hist
     static constexpr uint32_t kHistEntries = 65536u + 64u;
@@ -1082,6 +1084,7 @@ struct SmallSolver {
         ring_a = ctl_a + 128;
         gring = (glb_u16 *)(slot + P.garena_off);
         hist = (glb_u16 *)(slot + P.hist_off);
+        himg = (glb_u32 *)(slot + P.himg_off);
         goodp = (glb_u8 *)(slot + P.good_off);
         lvlp = (glb_u32 *)(slot + P.lvl_off);
         choicep = (glb_u32 *)(slot + P.gbitmap_off);
@@ -1199,8 +1202,8 @@ struct SmallSolver {
     // a singleton, kNoSync when Q is the only reachable subset (both letters
     // permutations, or n = 1: not synchronizing, one subset expanded), else
     // kContinue with level 1 in the ring.
-    __device__ int begin() { return start(); }
-    __device__ int start() {
+    __device__ __forceinline__ int begin() { return start(); }
+    __device__ __forceinline__ int start() {
         ai = nx_ai;
         seg = nx_seg;
         n = uni<G>(nx_n);
@@ -1409,7 +1412,7 @@ struct SmallSolver {
         in_lds = false;
     }
 
-    __device__ int step() {
+    __device__ __forceinline__ int step() {
         if constexpr (G > 64) return step_level();
         if constexpr (T == 1 && G == 64) {
             // one entry per lane, one wavefront, LDS ring: the steps of the
@@ -1682,6 +1685,13 @@ struct SmallSolver {
         bool nw[2 * TT];
         uint64_t m[2 * TT], sm;
         expand<TT>(img, v, nw, m, sm);
+        if constexpr (WIT && T >= 4) {
+            // the images of this step's entries at their history positions
+            // (the backward pass of the witness reads them there)
+#pragma unroll
+            for (int t = 0; t < TT; ++t)
+                if (v[t]) himg[SSW_IDX(rd + t * G + r, kHistEntries)] = img[t];
+        }
         bool any_single;
         uint32_t pos[2 * TT], total;
         g.template place<2 * TT>(sm, m, wr, any_single, pos, total);
@@ -2061,6 +2071,235 @@ struct SmallSolver {
         return true;
     }
 
+    // The passes of witness_lists for the long searches' launch (four entries
+    // per lane), with less LDS work and fewer dependent LDS round trips per
+    // level.  Backward: the history and the images the search recorded for
+    // it are read in rounds of four entries per lane (from the level's start
+    // rounded down to four entries), kAhead rounds ahead of the one being
+    // searched and across level boundaries (the loads do not depend on the
+    // marks), so below level L - 1 a subset costs two bitmap reads and no
+    // table lookup, and a round's reads are issued together.  Each good
+    // subset x is listed as x | ga << 16, ga = 1 when a(x) is good one level
+    // further (at level L - 1: a singleton), which is the forward greedy's
+    // choice at x.  A level's marks are set once its last round is done
+    // (from the round's registers when it had one round, else from its
+    // list) without waiting on them: the wave's LDS operations complete in
+    // order, so the next level's reads see them.  Level starts come 64 at a
+    // time.  Forward: at cur of level k >= 1 the letter is the ga of cur in
+    // G_k, found by one read per lane of the list below its count (issued
+    // with cur's image lookup), so a letter costs one round trip.  Returns
+    // false when the lists outgrow the area.
+    __device__ bool witness_long(int32_t L_in, const WitnessSpace &w) {
+        static_assert(BT, "the long searches' launch has byte tables");
+        // uniform (the compiler cannot tell): the passes' control flow is then scalar
+        const int32_t L = (int32_t)__builtin_amdgcn_readfirstlane((uint32_t)L_in);
+        constexpr uint32_t RW = 4u * G;  // entries per round
+        constexpr int kAhead = 4;        // rounds loaded ahead
+        const uint32_t r = g.rank();
+        const bool cached = __builtin_amdgcn_readfirstlane(w.starts_cached ? 1u : 0u) != 0u;
+        // Level starts, 64 levels per read: a window holds in lane i the
+        // start of level top - i (from the LDS copy, or else from HBM), and
+        // a start inside it is one lane read.  The two passes over the
+        // levels (the loads' cursor and the search) go down with windows of
+        // their own, so each level costs no wait on a start.
+        struct StartWindow {
+            uint32_t v;
+            int32_t top;
+        };
+        const auto window = [&](int32_t top) -> StartWindow {
+            const int32_t k = top - (int32_t)r;
+            uint32_t v = 0;
+            if (cached) {
+                v = k >= 0 ? (uint32_t)*l32(w.starts_a + 4u * (uint32_t)max(k, 0)) : 0u;
+            } else {
+                v = k >= 0 ? (uint32_t)lvlp[SSW_IDX((uint32_t)max(k, 0), kLevelEntries)] : 0u;
+            }
+            return StartWindow{v, top};
+        };
+        const auto start_in = [&](StartWindow &sw, int32_t k) -> uint32_t {
+            if (k > sw.top || k <= sw.top - 64) sw = window(k);
+            return __builtin_amdgcn_readlane(sw.v, sw.top - k);
+        };
+        StartWindow fwin = window(L), cwin = fwin;
+        const auto start = [&](int32_t k) -> uint32_t { return start_in(cwin, k); };
+        const uint32_t starts_bytes = cached ? 4u * ((uint32_t)L + 1u) : 0u;
+        const uint32_t list_off = (starts_bytes + 3u) & ~3u;
+        const uint32_t list_a = ring_a + list_off;
+        // from uniform values only (ring_a lives in a VGPR), so that the room
+        // test below is a scalar branch
+        const uint32_t cap = (2u * (rcap + 64u) - list_off) / 4u;  // u32 entries
+        uint32_t top = 0;  // u32 entries in use (uniform)
+        // the loads' cursor: level fj (done below 1), its start fs and end
+        // fe, the next round's base fb (the history's and the images' slots
+        // are 256-byte aligned, so every round's loads are aligned: 8 bytes
+        // of subsets and 16 of their images per lane)
+        const glb_u64 *h64 = (const glb_u64 *)hist;
+        const __attribute__((address_space(1))) u32x4 *i128 = (const __attribute__((address_space(1))) u32x4 *)himg;
+        struct Round {
+            uint64_t x;  // four subsets
+            u32x4 im;    // their images (levels below L - 1: every subset there was expanded)
+        };
+        int32_t fj = L - 1;
+        uint32_t fs = start_in(fwin, fj), fe = start_in(fwin, fj + 1), fb = fs & ~3u;
+        const auto fetch = [&]() -> Round {
+            // the loads are issued on every call, past the last level too
+            // (at the first words): the same number of loads per round, so
+            // the waits for a slot count exactly the rounds issued after it
+            const uint32_t q = fj >= 1 ? min(fb + 4u * r, kHistEntries - 4u) >> 2 : 0u;
+            Round v;
+            v.x = h64[SSW_IDX(q, kHistEntries / 4u)];
+            v.im = i128[SSW_IDX(q, kHistEntries / 4u)];
+            if (fj >= 1) {
+                fb += RW;
+                if (fb >= fe) {
+                    fe = fs;
+                    if (--fj >= 1) {
+                        fs = start_in(fwin, fj);
+                        fb = fs & ~3u;
+                    }
+                }
+            }
+            return v;
+        };
+        // slot d holds the round kAhead rounds after the one it last held
+        Round ahead[kAhead];
+#pragma unroll
+        for (int d = 0; d < kAhead; ++d) ahead[d] = fetch();
+        // the searching cursor: level j, its start s and end e, round base,
+        // rounds taken in the level, the first list entry of the level
+        int32_t j = L - 1;
+        uint32_t e = start(L), s = start(L - 1);
+        uint32_t base = s & ~3u, rounds = 0, first = 0;
+        uint32_t x[4] = {}, ent[4] = {};
+        bool on[4] = {};
+        while (j >= 1) {
+#pragma unroll
+            for (int d = 0; d < kAhead; ++d) {
+                if (j < 1) break;
+                // slot d is refilled once its round's values are taken out
+                // of it, so the refill can land in the same registers (no
+                // copy of a load in flight at the loop's back edge)
+                const Round v = ahead[d];
+                const bool last = j == L - 1;
+                uint32_t im[4] = {v.im.x, v.im.y, v.im.z, v.im.w};
+                bool ga[4], gb[4];
+#pragma unroll
+                for (uint32_t u = 0; u < 4u; ++u) x[u] = (uint32_t)(v.x >> (16u * u)) & 0xFFFFu;
+                if (last) {
+                    // level L - 1 (the search stopped inside it): images from
+                    // the tables, good = an image is a singleton
+                    uint32_t lo[4], hi[4];
+#pragma unroll
+                    for (uint32_t u = 0; u < 4u; ++u) {
+                        lo[u] = *l32(tab_a | ((x[u] << 2) & 0x3FCu));
+                        hi[u] = *l32((tab_a | ((x[u] >> 6) & 0x3FCu)) + 1024u);
+                    }
+#pragma unroll
+                    for (uint32_t u = 0; u < 4u; ++u) {
+                        im[u] = lo[u] | hi[u];
+                        ga[u] = is_single(im[u] & 0xFFFFu);
+                        gb[u] = is_single(im[u] >> 16);
+                    }
+                } else {
+                    // the recorded images' eight bitmap reads issued together
+                    // (nothing moved across the barrier: the kernel's register
+                    // pressure would otherwise pull each read next to its use)
+                    uint32_t wa[4], wb[4];
+#pragma unroll
+                    for (uint32_t u = 0; u < 4u; ++u) {
+                        wa[u] = *l32(bm_a | ((im[u] >> 3) & 0x1FFCu));
+                        wb[u] = *l32(bm_a | ((im[u] >> 19) & 0x1FFCu));
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                    for (uint32_t u = 0; u < 4u; ++u) {
+                        ga[u] = ((wa[u] >> (im[u] & 31u)) & 1u) != 0u;
+                        gb[u] = ((wb[u] >> ((im[u] >> 16) & 31u)) & 1u) != 0u;
+                    }
+                }
+                ahead[d] = fetch();
+                uint64_t m[4];
+                uint32_t total = 0;
+#pragma unroll
+                for (uint32_t u = 0; u < 4u; ++u) {
+                    const uint32_t p = base + 4u * r + u;
+                    on[u] = (p >= s) & (p < e) & (ga[u] | gb[u]);
+                    ent[u] = x[u] | (ga[u] ? 0x10000u : 0u);
+                    m[u] = __ballot(on[u]);
+                    total += (uint32_t)__popcll(m[u]);
+                }
+                if (top + total >= cap) return false;  // no room for them and the count
+#pragma unroll
+                for (uint32_t u = 0; u < 4u; ++u) {
+                    if (on[u])
+                        *l32(list_a + 4u * __builtin_amdgcn_mbcnt_hi((uint32_t)(m[u] >> 32),
+                                                                   __builtin_amdgcn_mbcnt_lo((uint32_t)m[u], top))) =
+                            ent[u];
+                    top += (uint32_t)__popcll(m[u]);
+                }
+                ++rounds;
+                base += RW;
+                if (base >= e) {
+                    // level j done: its count, then its marks
+                    if (r == 0) *l32(list_a + 4u * top) = top - first;
+                    ++top;
+                    if (rounds == 1) {
+                        // every good subset of the level is in this round's registers
+#pragma unroll
+                        for (uint32_t u = 0; u < 4u; ++u)
+                            if (on[u]) or_wg(l32(bm_a | ((x[u] >> 3) & 0x1FFCu)), 1u << (x[u] & 31u));
+                    } else {
+                        lsync();
+                        for (uint32_t q = first + r; q + 1u < top; q += G) {
+                            const uint32_t y = *l32(list_a + 4u * q) & 0xFFFFu;
+                            or_wg(l32(bm_a | ((y >> 3) & 0x1FFCu)), 1u << (y & 31u));
+                        }
+                    }
+                    lsync();
+                    --j;
+                    e = s;
+                    if (j >= 1) s = start(j);
+                    base = s & ~3u;
+                    rounds = 0;
+                    first = top;
+                }
+            }
+        }
+        glb_u8 *out = row();
+        uint32_t cur = Q, acc = 0, endp = top;
+        for (int32_t k = 0; k < L; ++k) {
+            bool take_a;
+            const uint32_t im = image(cur);
+            if (k == 0) {
+                take_a = L == 1 ? is_single(im & 0xFFFFu) : marked(im & 0xFFFFu);  // level 1's marks
+            } else {
+                // G_k: its count at endp - 1, its entries below it; lane 0
+                // reads the count, lane i the i-th entry down (one read
+                // covers lists of up to 63 entries)
+                const uint32_t v0 = r < endp ? *l32(list_a + 4u * (endp - 1u - r)) : 0u;
+                const uint32_t c = __builtin_amdgcn_readfirstlane(v0);
+                uint64_t hit = __ballot(r >= 1u && r <= c && (v0 & 0xFFFFu) == cur);
+                uint32_t val = hit ? __builtin_amdgcn_readlane(v0, (int)__ffsll((long long)hit) - 1) : 0u;
+                for (uint32_t q0 = 64u; !hit && q0 <= c; q0 += G) {
+                    const uint32_t q = q0 + r;
+                    const uint32_t vq = q <= c ? *l32(list_a + 4u * (endp - 1u - q)) : 0u;
+                    hit = __ballot(q <= c && (vq & 0xFFFFu) == cur);
+                    if (hit) val = __builtin_amdgcn_readlane(vq, (int)__ffsll((long long)hit) - 1);
+                }
+                take_a = (val >> 16) != 0u;
+                endp -= c + 1u;
+            }
+            if (!take_a) acc |= 1u << (k & 7);
+            cur = take_a ? (im & 0xFFFFu) : (im >> 16);
+            if ((k & 7) == 7 || k == L - 1) {
+                if (r == 0 && (k >> 3) < P.witness_stride) out[k >> 3] = (uint8_t)acc;
+                acc = 0;
+            }
+        }
+        zero_row((L + 7) >> 3);
+        return true;
+    }
+
     // The passes for a history in HBM (a long search: its level chain is the
     // cost, so round trips per level are what is cut).  First with the good
     // subsets as lists (witness_lists); when those outgrow their room, again
@@ -2076,7 +2315,11 @@ struct SmallSolver {
     // marks of level 1, then the choice bits, copied into the LDS bitmap, one
     // LDS read per letter (no search of a level's positions).
     __device__ void witness_hbm(int32_t L, const WitnessSpace &w) {
-        if (witness_lists(L, w)) return;
+        if constexpr (T >= 4) {
+            if (witness_long(L, w)) return;
+        } else {
+            if (witness_lists(L, w)) return;
+        }
         const uint32_t r = g.rank();
         lsync();
         clear_bitmap();
@@ -2116,7 +2359,7 @@ struct SmallSolver {
         zero_row((L + 7) >> 3);
     }
 
-    __device__ void finish(int code) {
+    __device__ __forceinline__ void finish(int code) {
         if (code == kSpilled) {
             lsync();
             return;

@@ -113,6 +113,7 @@ struct BatchParams {
     uint64_t slot_bytes;
     uint64_t garena_off;               // u32 frontier arena beyond the LDS part
     uint64_t hist_off;                 // n <= 16 with witness words: subsets in discovery order
+    uint64_t himg_off;                 // ... and (4 entries per lane) their images, packed as the tables'
     uint64_t garena_cap;               // entries
     uint64_t gbitmap_off;              // visited bitmap for n > lds_n
     uint64_t gbitmap_words;            // its size in u32 words (0: none)

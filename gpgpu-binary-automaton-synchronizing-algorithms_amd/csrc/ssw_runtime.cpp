@@ -535,6 +535,11 @@ int ssw_context::launch_pass(const DeviceBatch &db, const ClassSpec &cls, const 
             // bitmap (zeroed with the slot, left clean by every word)
             P.gbitmap_off = off;
             off = align_up(off + 4ull * bm_words, 256);
+            // the long searches' launch also keeps each expanded subset's
+            // images (u32 per history position), which its backward pass
+            // reads instead of looking them up again
+            P.himg_off = off;
+            if (T >= 4) off = align_up(off + 4ull * (65536 + 64), 256);
         } else {
             P.hist_off = P.gbitmap_off = P.good_off = P.lvl_off = off;
         }
