@@ -2180,6 +2180,10 @@ struct SmallSolver {
                 // of it, so the refill can land in the same registers (no
                 // copy of a load in flight at the loop's back edge)
                 const Round v = ahead[d];
+                // both halves of the slot consumed on every path (level L - 1
+                // reads no images): a load left pending on one path would
+                // make the slot's refill below wait for every load in flight
+                asm volatile("" ::"v"(v.x), "v"(v.im));
                 const bool last = j == L - 1;
                 uint32_t im[4] = {v.im.x, v.im.y, v.im.z, v.im.w};
                 bool ga[4], gb[4];
