@@ -205,6 +205,11 @@ struct ssw_context {
     hipEvent_t lev[2 * ssw::kMaxLaunchRecords] = {};
     std::vector<ssw::LaunchRecord> records;
     ssw::DevBuf d_delta, d_nper, d_off, d_len, d_sync, d_exp, d_wit;
+    // the second buffer set of the pipelined int32 path (chunk k uses set k & 1)
+    ssw::DevBuf d_delta2, d_len2, d_sync2, d_exp2;
+    // its copy stream's events: inputs of set s copied (0, 1), set s's
+    // kernels done with them (2, 3), outputs of set s copied back (4, 5)
+    hipEvent_t pipe_ev[6] = {};
     std::map<ssw::SlotKey, ssw::DevBuf> slot_cache;
     std::map<ssw::SlotKey, uint64_t> slot_gen;   // call that last used each entry
     uint64_t call_gen = 0;
@@ -252,13 +257,21 @@ struct ssw_context {
     // host_n: the sizes in host memory, or null for a uniform batch or when
     // size_hist (counts per n) is given; then the sizes are copied from the
     // device only where a route needs them one by one
+    // head_out (pinned, 16 bytes): the solve's counters are copied there
+    // behind its launches and the call returns without waiting (a uniform
+    // batch of n <= 16, whose automata are never deferred; the caller reads
+    // the violation once the stream is drained)
     int solve_device(const ssw::DeviceBatch &db, const uint8_t *host_n, bool plain, int64_t *violation_out,
-                     const int64_t *size_hist = nullptr);
+                     const int64_t *size_hist = nullptr, void *head_out = nullptr);
     int solve_host_packed(const ssw_batch *b, int mode, int64_t *violation_out);
     // host staging for the int32 entry points: pinned, grown on demand
     void *h_stage = nullptr;
     size_t h_stage_bytes = 0;
     int ensure_stage(size_t bytes);
+    // pinned outputs of the pipelined int32 path (two chunks' results and the heads)
+    void *h_out = nullptr;
+    size_t h_out_bytes = 0;
+    int ensure_out(size_t bytes);
     // set by compute_i32 for the call it makes: transitions already checked,
     // lengths not wanted by the caller (Verify)
     bool host_checked = false;

@@ -116,6 +116,7 @@ int ssw_context::open(int dev, uint64_t mem_budget) {
         HIP_TRY(hipStreamCreateWithFlags(&class_streams[k], hipStreamNonBlocking), "hipStreamCreate");
         HIP_TRY(hipEventCreateWithFlags(&ev_join[k], hipEventDisableTiming), "hipEventCreate");
     }
+    for (hipEvent_t &e : pipe_ev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
     size_t free_b = 0, total_b = 0;
     HIP_TRY(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
     budget = mem_budget ? mem_budget : (uint64_t)(0.8 * (double)free_b);
@@ -143,7 +144,7 @@ void ssw_context::close() {
     stop_progress_thread();
     if (device >= 0) hipSetDevice(device);
     for (DevBuf *b : {&ctrs, &queues, &deferred, &order, &spill, &lstats, &prog, &sizes, &d_delta, &d_nper, &d_off, &d_len,
-                      &d_sync, &d_exp, &d_wit})
+                      &d_sync, &d_exp, &d_wit, &d_delta2, &d_len2, &d_sync2, &d_exp2})
         if (b->p) hipFree(b->p);
     for (int k = 0; k < 2 * kMaxLaunchRecords; ++k)
         if (lev[k]) hipEventDestroy(lev[k]);
@@ -160,8 +161,12 @@ void ssw_context::close() {
     }
     if (stream) hipStreamDestroy(stream);
     if (monitor) hipStreamDestroy(monitor);
+    for (hipEvent_t &e : pipe_ev)
+        if (e) hipEventDestroy(e);
     if (h_stage) hipHostFree(h_stage);
     h_stage = nullptr;
+    if (h_out) hipHostFree(h_out);
+    h_out = nullptr;
 }
 
 void ssw_context::default_tuning() {
@@ -652,7 +657,7 @@ int ssw_context::launch_pass(const DeviceBatch &db, const ClassSpec &cls, const 
 
 // Splits the batch into launch classes: (thread-group width, bitmap in LDS).
 int ssw_context::solve_device(const DeviceBatch &db, const uint8_t *host_n, bool plain,
-                              int64_t *violation_out, const int64_t *size_hist) {
+                              int64_t *violation_out, const int64_t *size_hist, void *head_out) {
     // the sizes one by one, copied from the device when a route needs them
     std::vector<uint8_t> fetched_n;
     auto need_host_n = [&]() -> int {
@@ -834,6 +839,11 @@ int ssw_context::solve_device(const DeviceBatch &db, const uint8_t *host_n, bool
     // the deferred count and the violation in one read: automata whose
     // frontier outgrew a group's arena go on to the device-wide path
     static_assert(kCtrDeferred + 4 <= kCtrViolation && kCtrViolation + 8 <= 16, "counter layout");
+    if (head_out) {
+        HIP_TRY(hipMemcpyAsync(head_out, ctrs.p, 16, hipMemcpyDeviceToHost, stream), "D2H");
+        last_launches = launches;
+        return SSW_OK;
+    }
     alignas(8) uint8_t head[16];
     HIP_TRY(hipMemcpyAsync(head, ctrs.p, sizeof head, hipMemcpyDeviceToHost, stream), "D2H");
     HIP_TRY(hipStreamSynchronize(stream), "hipStreamSynchronize");
@@ -987,6 +997,17 @@ int ssw_context::solve_host_packed(const ssw_batch *b, int mode, int64_t *violat
                                hipMemcpyDeviceToHost, stream),
                 "D2H");
     HIP_TRY(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    return SSW_OK;
+}
+
+int ssw_context::ensure_out(size_t bytes) {
+    if (h_out && h_out_bytes >= bytes) return SSW_OK;
+    if (h_out) hipHostFree(h_out);
+    h_out = nullptr;
+    h_out_bytes = 0;
+    const size_t want = std::max<size_t>(bytes, 1u << 20);
+    HIP_TRY(hipHostMalloc(&h_out, want, hipHostMallocDefault), "hipHostMalloc");
+    h_out_bytes = want;
     return SSW_OK;
 }
 
@@ -1247,15 +1268,139 @@ int ssw_context::compute_i32(int32_t n, int64_t count, const int32_t *A, const i
     };
     static int32_t no_length;
     int64_t viol = -1;
+    const auto bad_input = [&](int64_t first) {
+        int q = 0;
+        while (q < n && (uint32_t)A[first * n + q] < (uint32_t)n && (uint32_t)B[first * n + q] < (uint32_t)n) ++q;
+        return fail(SSW_ERR_ARG, "automaton %lld state %d: transition out of [0,%d)", (long long)first, q, n);
+    };
+    if (nchunks > 1 && !witness_out && n <= kMaxLdsN) {
+        // Pipelined: chunk k's transitions are copied on the copy stream while
+        // chunk k - 1's launches run, into the device set k & 1; its results
+        // come back into pinned memory on a third stream once its launches
+        // are done and are copied to the caller while chunk k + 1 runs.  The host waits only for packed
+        // chunks and finished results; the solves' counters (n <= 16: no
+        // automaton is deferred) are read once at the end.
+        const uint64_t most = (uint64_t)((count + nchunks - 1) / nchunks);
+        const uint64_t out_row = 4u + (sync_out ? 1u : 0u) + (expanded_out ? 8u : 0u);
+        const uint64_t half = align_up(most * out_row + 16, 256);
+        if ((rc = ensure(d_delta2, most * 2u * (uint64_t)n, "delta"))) return rc;
+        if ((rc = ensure(d_len2, 4ull * most, "length"))) return rc;
+        if (sync_out && (rc = ensure(d_sync2, most, "sync"))) return rc;
+        if (expanded_out && (rc = ensure(d_exp2, 8ull * most, "expanded"))) return rc;
+        if ((rc = ensure_out(2 * half + 16u * (uint64_t)nchunks))) return rc;
+        uint8_t *heads = (uint8_t *)h_out + 2 * half;
+        // inputs and outputs on streams of their own, so that neither waits
+        // behind the other's dependencies
+        hipStream_t copy = class_streams[kClassStreams - 1], back = class_streams[kClassStreams - 2];
+        // host part of a set's results: lengths, then flags, then counts
+        const auto out_len = [&](int s) { return (int32_t *)((uint8_t *)h_out + s * half); };
+        const auto out_sync = [&](int s) { return (uint8_t *)(out_len(s) + most); };
+        const auto out_exp = [&](int s) {
+            return (int64_t *)((uint8_t *)h_out + s * half + align_up(most * (4u + (sync_out ? 1u : 0u)), 8));
+        };
+        const auto deliver = [&](int64_t k) -> int {  // chunk k's results to the caller
+            const int s = (int)(k & 1);
+            HIP_TRY(hipEventSynchronize(pipe_ev[4 + s]), "hipEventSynchronize");
+            const int64_t lo = chunk_lo(k), len = chunk_lo(k + 1) - lo;
+            if (length_out) std::memcpy(length_out + lo, out_len(s), 4ull * len);
+            if (sync_out) std::memcpy(sync_out + lo, out_sync(s), (size_t)len);
+            if (expanded_out) std::memcpy(expanded_out + lo, out_exp(s), 8ull * len);
+            return SSW_OK;
+        };
+        const auto drain = [&](int code) {
+            abort_pack.store(true);
+            join_all();
+            hipStreamSynchronize(copy);
+            hipStreamSynchronize(stream);
+            hipStreamSynchronize(back);
+            solve_end();
+            return code;
+        };
+        HIP_TRY(hipSetDevice(device), "hipSetDevice");
+        solve_begin((uint64_t)count);
+        host_no_length = length_out == nullptr;
+        for (int64_t k = 0; k < nchunks; ++k) {
+            while (packed_parts[k].load(std::memory_order_acquire) < workers) std::this_thread::yield();
+            if (bad[k].load() >= 0) {
+                const int64_t first = bad[k].load();
+                host_no_length = false;
+                drain(0);
+                return bad_input(first);
+            }
+            const int s = (int)(k & 1);
+            const int64_t lo = chunk_lo(k), len = chunk_lo(k + 1) - lo;
+            DevBuf &dd = s ? d_delta2 : d_delta, &dl = s ? d_len2 : d_len, &ds = s ? d_sync2 : d_sync,
+                   &de = s ? d_exp2 : d_exp;
+            hipError_t e = hipSuccess;
+            // the set's inputs are overwritten once chunk k - 2's launches are done with them
+            if (k >= 2) e = hipStreamWaitEvent(copy, pipe_ev[2 + s], 0);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(dd.p, packed + (size_t)lo * 2 * n, (size_t)len * 2 * n, hipMemcpyHostToDevice, copy);
+            if (e == hipSuccess) e = hipEventRecord(pipe_ev[s], copy);
+            if (e == hipSuccess) e = hipStreamWaitEvent(stream, pipe_ev[s], 0);
+            // the set's results are overwritten once chunk k - 2's have been copied back
+            if (e == hipSuccess && k >= 2) e = hipStreamWaitEvent(stream, pipe_ev[4 + s], 0);
+            if (e != hipSuccess) {
+                host_no_length = false;
+                return drain(hip_fail(e, "pipelined chunk input"));
+            }
+            DeviceBatch db;
+            db.count = len;
+            db.uniform_n = n;
+            db.bound = bound;
+            db.delta = (const uint8_t *)dd.p;
+            db.length = (int32_t *)dl.p;
+            db.sync = sync_out ? (uint8_t *)ds.p : nullptr;
+            db.expanded = expanded_out ? (int64_t *)de.p : nullptr;
+            rc = solve_device(db, nullptr, false, nullptr, nullptr, heads + 16 * k);
+            if (rc) {
+                host_no_length = false;
+                return drain(rc);
+            }
+            e = hipEventRecord(pipe_ev[2 + s], stream);
+            if (e == hipSuccess) e = hipStreamWaitEvent(back, pipe_ev[2 + s], 0);
+            if (e == hipSuccess && length_out)
+                e = hipMemcpyAsync(out_len(s), dl.p, 4ull * len, hipMemcpyDeviceToHost, back);
+            if (e == hipSuccess && sync_out) e = hipMemcpyAsync(out_sync(s), ds.p, (size_t)len, hipMemcpyDeviceToHost, back);
+            if (e == hipSuccess && expanded_out)
+                e = hipMemcpyAsync(out_exp(s), de.p, 8ull * len, hipMemcpyDeviceToHost, back);
+            if (e == hipSuccess) e = hipEventRecord(pipe_ev[4 + s], back);
+            if (e == hipSuccess && k >= 1) rc = deliver(k - 1);
+            if (e != hipSuccess || rc) {
+                host_no_length = false;
+                return drain(e != hipSuccess ? hip_fail(e, "pipelined chunk output") : rc);
+            }
+        }
+        host_no_length = false;
+        rc = deliver(nchunks - 1);
+        hipError_t e = hipStreamSynchronize(stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(back);
+        join_all();
+        solve_end();
+        if (rc) return rc;
+        if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, ev0, ev1) == hipSuccess) last_ms = ms;  // the last chunk's kernels
+        for (int64_t k = 0; k < nchunks; ++k) {
+            uint32_t ndef = 0;
+            unsigned long long v = ~0ull;
+            std::memcpy(&ndef, heads + 16 * k + kCtrDeferred, 4);
+            std::memcpy(&v, heads + 16 * k + kCtrViolation, 8);
+            if (ndef) return fail(SSW_ERR_INTERNAL, "an automaton of n=%d was deferred", n);
+            if (v != ~0ull && (viol < 0 || chunk_lo(k) + (int64_t)v < viol)) viol = chunk_lo(k) + (int64_t)v;
+        }
+        const int crc = checked_verdict();
+        if (crc) return crc;
+        if (violation_out) *violation_out = viol;
+        return SSW_OK;
+    }
     for (int64_t k = 0; k < nchunks; ++k) {
         while (packed_parts[k].load(std::memory_order_acquire) < workers) std::this_thread::yield();
         const int64_t first = bad[k].load();
         if (first >= 0) {
             abort_pack.store(true);
             join_all();
-            int q = 0;
-            while (q < n && (uint32_t)A[first * n + q] < (uint32_t)n && (uint32_t)B[first * n + q] < (uint32_t)n) ++q;
-            return fail(SSW_ERR_ARG, "automaton %lld state %d: transition out of [0,%d)", (long long)first, q, n);
+            return bad_input(first);
         }
         const int64_t lo = chunk_lo(k), len = chunk_lo(k + 1) - lo;
         ssw_batch b;

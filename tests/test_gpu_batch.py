@@ -365,3 +365,56 @@ def test_host_arrays_chunked_n20(ctx):
     np.testing.assert_array_equal(r.length, L)
     np.testing.assert_array_equal(r.sync, S)
     np.testing.assert_array_equal(r.expanded, E)
+
+
+def test_host_arrays_pipelined_n16(ctx):
+    """The pipelined host path (n <= 16, two or more chunks: the next chunk's
+    transitions copied under the current chunk's launches, results copied
+    back on their own stream) against the oracle, with Cerny C_16 stragglers
+    in both chunks; lengths only as Verify asks for them; and the violation
+    index of Verify when every violation lies in the second chunk."""
+    n = 16
+    count = 600_000  # two chunks of 300,000
+    A, B = ssw.generate_uniform(1617, n, count)
+    ca, cb = ssw.generate_cerny(n)
+    for i in (7, 123_456, 299_999, 300_000, 450_001, count - 1):
+        A[i], B[i] = ca, cb
+    print("pipelined n=16: oracle", flush=True)
+    L, S, E, _ = O.solve(n, A, B, threads=8)
+    print("pipelined n=16: compute", flush=True)
+    r = ctx.compute(n, A, B)
+    np.testing.assert_array_equal(r.length, L)
+    np.testing.assert_array_equal(r.sync, S)
+    np.testing.assert_array_equal(r.expanded, E)
+    assert r.violation == -1
+    r2 = ctx.compute(n, A, B, want_sync=False, want_expanded=False)
+    np.testing.assert_array_equal(r2.length, L)
+    # a bound every automaton of the first chunk keeps: the first violation
+    # (C_16 at 300,000 is the first automaton above the bound) is found in
+    # the second chunk and reported at its index in the whole call
+    bound = int(L[:300_000][L[:300_000] != 225].max())
+    A2, B2 = A.copy(), B.copy()
+    for i in (7, 123_456, 299_999):
+        A2[i], B2[i] = A[0], B[0]
+    L2 = L.copy()
+    L2[[7, 123_456, 299_999]] = L[0]
+    expected = int(np.nonzero(L2 > bound)[0][0])
+    assert expected >= 300_000
+    assert ctx.verify(n, A2, B2, bound=bound) == expected
+    assert ctx.compute(n, A2, B2, bound=bound).violation == expected
+
+
+def test_host_arrays_pipelined_bad_input_late(ctx):
+    """A transition out of range in the second chunk, found while the first
+    chunk is already on the device: the call fails naming that automaton,
+    drains its work and leaves the context usable."""
+    n = 13
+    count = 560_000
+    A, B = ssw.generate_uniform(1618, n, count)
+    A[400_123, 5] = n
+    with pytest.raises(ssw.SswError, match="400123"):
+        ctx.compute(n, A, B)
+    A[400_123, 5] = 0
+    r = ctx.compute(n, A[:1000], B[:1000])
+    L, _, _, _ = O.solve(n, A[:1000], B[:1000], threads=8)
+    np.testing.assert_array_equal(r.length, L)
