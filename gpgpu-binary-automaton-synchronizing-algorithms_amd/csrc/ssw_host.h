@@ -5,6 +5,7 @@
 #include "ssw_internal.h"
 
 #include <atomic>
+#include <functional>
 #include <condition_variable>
 #include <cstdarg>
 #include <map>
@@ -205,10 +206,10 @@ struct ssw_context {
     hipEvent_t lev[2 * ssw::kMaxLaunchRecords] = {};
     std::vector<ssw::LaunchRecord> records;
     ssw::DevBuf d_delta, d_nper, d_off, d_len, d_sync, d_exp, d_wit;
-    // the second buffer set of the pipelined int32 path (chunk k uses set k & 1)
+    // the second buffer set of the host pipeline (chunk k uses set k & 1)
     ssw::DevBuf d_delta2, d_len2, d_sync2, d_exp2;
-    // its copy stream's events: inputs of set s copied (0, 1), set s's
-    // kernels done with them (2, 3), outputs of set s copied back (4, 5)
+    // its streams' events: inputs of set s copied (0, 1), set s's kernels
+    // done with them (2, 3), outputs of set s copied back (4, 5)
     hipEvent_t pipe_ev[6] = {};
     std::map<ssw::SlotKey, ssw::DevBuf> slot_cache;
     std::map<ssw::SlotKey, uint64_t> slot_gen;   // call that last used each entry
@@ -264,11 +265,17 @@ struct ssw_context {
     int solve_device(const ssw::DeviceBatch &db, const uint8_t *host_n, bool plain, int64_t *violation_out,
                      const int64_t *size_hist = nullptr, void *head_out = nullptr);
     int solve_host_packed(const ssw_batch *b, int mode, int64_t *violation_out);
+    // a uniform batch of n <= 16 from host memory in pipelined chunks (fill
+    // packs automata [lo, hi) into the pinned staging and returns the first
+    // invalid one, or -1; bad_at reports it)
+    int host_pipeline(int32_t n, int64_t count, int64_t nchunks, const std::function<int64_t(int64_t, int64_t)> &fill,
+                      const std::function<int(int64_t)> &bad_at, int32_t *length_out, uint8_t *sync_out,
+                      int64_t *expanded_out, int32_t bound, int64_t *violation_out);
     // host staging for the int32 entry points: pinned, grown on demand
     void *h_stage = nullptr;
     size_t h_stage_bytes = 0;
     int ensure_stage(size_t bytes);
-    // pinned outputs of the pipelined int32 path (two chunks' results and the heads)
+    // pinned results of the host pipeline (two chunks' results and the solves' counters)
     void *h_out = nullptr;
     size_t h_out_bytes = 0;
     int ensure_out(size_t bytes);

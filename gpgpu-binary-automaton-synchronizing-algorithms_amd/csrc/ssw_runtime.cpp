@@ -227,6 +227,12 @@ int ssw_context::get_slots(const SlotKey &key, uint64_t bytes, DevBuf **out, hip
 
 static inline uint64_t align_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
 
+// host calls cut into chunks: few large ones, since every solve pays its
+// launches' tails (the longest searches of the chunk), so a chunk below
+// ~2^18 automata costs more in tails than the host work and copies it hides
+static constexpr int64_t kMinHostChunk = 1 << 18;
+
+
 static inline uint32_t next_pow2(uint64_t x) {
     uint64_t p = 1;
     while (p < x) p <<= 1;
@@ -917,6 +923,33 @@ int ssw_context::ensure_stage(size_t bytes) {
 
 int ssw_context::solve_host_packed(const ssw_batch *b, int mode, int64_t *violation_out) {
     const int64_t count = b->count;
+    const int64_t nchunks = std::min<int64_t>(4, count / kMinHostChunk);
+    if (!host_checked && mode == kModeBatch && b->uniform_n > 0 && b->uniform_n <= kMaxLdsN && !b->offset &&
+        !b->witness && nchunks > 1) {
+        // the caller's rows (pageable memory) copied into pinned staging and
+        // checked by host threads chunk by chunk, under the transfers and
+        // kernels of the chunks before (host_pipeline)
+        const int32_t n = b->uniform_n;
+        int rc = ensure_stage((size_t)count * 2 * n);
+        if (rc) return rc;
+        uint8_t *packed = (uint8_t *)h_stage;
+        return host_pipeline(
+            n, count, nchunks,
+            [&](int64_t lo, int64_t hi) -> int64_t {
+                const size_t row = 2u * (size_t)n;
+                const uint8_t *src = b->delta + (size_t)lo * row;
+                std::memcpy(packed + (size_t)lo * row, src, (size_t)(hi - lo) * row);
+                uint8_t most = 0;
+                for (size_t q = 0, e = (size_t)(hi - lo) * row; q < e; ++q) most = std::max(most, src[q]);
+                if (most < (uint8_t)n) return -1;
+                for (int64_t i = lo; i < hi; ++i)
+                    for (size_t q = 0; q < row; ++q)
+                        if (b->delta[(size_t)i * row + q] >= n) return i;
+                return -1;
+            },
+            [&](int64_t first) { return fail(SSW_ERR_ARG, "automaton %lld: a transition >= n=%d", (long long)first, n); },
+            host_no_length ? nullptr : b->length, b->sync, b->expanded, b->bound, violation_out);
+    }
     // extent of the packed transitions and validation (first bad automaton)
     uint64_t extent = 0;
     if (host_checked && b->uniform_n > 0 && !b->offset) {
@@ -1205,6 +1238,156 @@ static int64_t pack_range(int32_t n, const int32_t *A, const int32_t *B, uint8_t
     return -1;
 }
 
+// A uniform batch of n <= 16 from host memory as a pipeline of `nchunks`
+// chunks.  Host threads fill the pinned staging buffer chunk by chunk
+// (fill(lo, hi) writes the packed rows of automata [lo, hi) and returns the
+// first invalid one there, or -1); chunk k's rows are copied on a copy stream
+// while chunk k - 1's launches run, into the device set k & 1; its results
+// come back into pinned memory on a third stream once its launches are done
+// and are copied to the caller while chunk k + 1 runs.  The host waits only
+// for filled chunks and finished results; the solves' counters (n <= 16: no
+// automaton is deferred) are read once at the end.  An invalid automaton
+// drains the work already queued and returns bad_at(index).
+// (Solving the whole batch once after pipelined copies instead was measured:
+// the 1M n = 16 batch gains little, since its host packing is as long as the
+// copies, while the n = 13 Verify workload loses the overlap of its packing
+// with the solves: 3.3 -> 4.2 ms.)
+int ssw_context::host_pipeline(int32_t n, int64_t count, int64_t nchunks,
+                               const std::function<int64_t(int64_t, int64_t)> &fill,
+                               const std::function<int(int64_t)> &bad_at, int32_t *length_out, uint8_t *sync_out,
+                               int64_t *expanded_out, int32_t bound, int64_t *violation_out) {
+    if (violation_out) *violation_out = -1;
+    uint8_t *packed = (uint8_t *)h_stage;
+    auto chunk_lo = [&](int64_t k) { return count * k / nchunks; };
+    const int64_t hw = std::max<int64_t>(1, (int64_t)std::thread::hardware_concurrency());
+    const int workers = (int)std::max<int64_t>(1, std::min<int64_t>(hw, 16) - 1);
+    std::vector<std::atomic<int>> filled(nchunks);
+    std::vector<std::atomic<int64_t>> bad(nchunks);
+    for (int64_t k = 0; k < nchunks; ++k) {
+        filled[k].store(0);
+        bad[k].store(-1);
+    }
+    const uint64_t most = (uint64_t)((count + nchunks - 1) / nchunks);
+    const uint64_t out_row = 4u + (sync_out ? 1u : 0u) + (expanded_out ? 8u : 0u);
+    const uint64_t half = align_up(most * out_row + 16, 256);
+    int rc;
+    if ((rc = ensure(d_delta, most * 2u * (uint64_t)n, "delta"))) return rc;
+    if ((rc = ensure(d_len, 4ull * most, "length"))) return rc;
+    if (sync_out && (rc = ensure(d_sync, most, "sync"))) return rc;
+    if (expanded_out && (rc = ensure(d_exp, 8ull * most, "expanded"))) return rc;
+    if ((rc = ensure(d_delta2, most * 2u * (uint64_t)n, "delta"))) return rc;
+    if ((rc = ensure(d_len2, 4ull * most, "length"))) return rc;
+    if (sync_out && (rc = ensure(d_sync2, most, "sync"))) return rc;
+    if (expanded_out && (rc = ensure(d_exp2, 8ull * most, "expanded"))) return rc;
+    if ((rc = ensure_out(2 * half + 16u * (uint64_t)nchunks))) return rc;
+    uint8_t *heads = (uint8_t *)h_out + 2 * half;
+    // inputs and outputs on streams of their own, so that neither waits
+    // behind the other's dependencies
+    hipStream_t copy = class_streams[kClassStreams - 1], back = class_streams[kClassStreams - 2];
+    // host part of a set's results: lengths, then flags, then counts
+    const auto out_len = [&](int s) { return (int32_t *)((uint8_t *)h_out + s * half); };
+    const auto out_sync = [&](int s) { return (uint8_t *)(out_len(s) + most); };
+    const auto out_exp = [&](int s) {
+        return (int64_t *)((uint8_t *)h_out + s * half + align_up(most * (4u + (sync_out ? 1u : 0u)), 8));
+    };
+    std::atomic<bool> stop{false};
+    auto work = [&](int t) {
+        for (int64_t k = 0; k < nchunks && !stop.load(std::memory_order_relaxed); ++k) {
+            const int64_t lo = chunk_lo(k), len = chunk_lo(k + 1) - lo;
+            const int64_t first = fill(lo + len * t / workers, lo + len * (t + 1) / workers);
+            if (first >= 0) {
+                int64_t cur = bad[k].load();
+                while ((cur < 0 || first < cur) && !bad[k].compare_exchange_weak(cur, first)) {
+                }
+            }
+            filled[k].fetch_add(1, std::memory_order_release);
+        }
+    };
+    std::vector<std::thread> ts;
+    for (int t = 0; t < workers; ++t) ts.emplace_back(work, t);
+    const auto join_all = [&] {
+        for (auto &th : ts) th.join();
+        ts.clear();
+    };
+    const auto deliver = [&](int64_t k) -> int {  // chunk k's results to the caller
+        const int s = (int)(k & 1);
+        HIP_TRY(hipEventSynchronize(pipe_ev[4 + s]), "hipEventSynchronize");
+        const int64_t lo = chunk_lo(k), len = chunk_lo(k + 1) - lo;
+        if (length_out) std::memcpy(length_out + lo, out_len(s), 4ull * len);
+        if (sync_out) std::memcpy(sync_out + lo, out_sync(s), (size_t)len);
+        if (expanded_out) std::memcpy(expanded_out + lo, out_exp(s), 8ull * len);
+        return SSW_OK;
+    };
+    const auto drain = [&](int code) {
+        stop.store(true);
+        join_all();
+        hipStreamSynchronize(copy);
+        hipStreamSynchronize(stream);
+        hipStreamSynchronize(back);
+        return code;
+    };
+    for (int64_t k = 0; k < nchunks; ++k) {
+        while (filled[k].load(std::memory_order_acquire) < workers) std::this_thread::yield();
+        if (bad[k].load() >= 0) {
+            const int64_t first = bad[k].load();
+            drain(0);
+            return bad_at(first);
+        }
+        const int s = (int)(k & 1);
+        const int64_t lo = chunk_lo(k), len = chunk_lo(k + 1) - lo;
+        DevBuf &dd = s ? d_delta2 : d_delta, &dl = s ? d_len2 : d_len, &ds = s ? d_sync2 : d_sync,
+               &de = s ? d_exp2 : d_exp;
+        hipError_t e = hipSuccess;
+        // the set's inputs are overwritten once chunk k - 2's launches are done with them
+        if (k >= 2) e = hipStreamWaitEvent(copy, pipe_ev[2 + s], 0);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(dd.p, packed + (size_t)lo * 2 * n, (size_t)len * 2 * n, hipMemcpyHostToDevice, copy);
+        if (e == hipSuccess) e = hipEventRecord(pipe_ev[s], copy);
+        if (e == hipSuccess) e = hipStreamWaitEvent(stream, pipe_ev[s], 0);
+        // the set's results are overwritten once chunk k - 2's have been copied back
+        if (e == hipSuccess && k >= 2) e = hipStreamWaitEvent(stream, pipe_ev[4 + s], 0);
+        if (e != hipSuccess) return drain(hip_fail(e, "pipelined chunk input"));
+        DeviceBatch db;
+        db.count = len;
+        db.uniform_n = n;
+        db.bound = bound;
+        db.delta = (const uint8_t *)dd.p;
+        db.length = (int32_t *)dl.p;
+        db.sync = sync_out ? (uint8_t *)ds.p : nullptr;
+        db.expanded = expanded_out ? (int64_t *)de.p : nullptr;
+        rc = solve_device(db, nullptr, false, nullptr, nullptr, heads + 16 * k);
+        if (rc) return drain(rc);
+        e = hipEventRecord(pipe_ev[2 + s], stream);
+        if (e == hipSuccess) e = hipStreamWaitEvent(back, pipe_ev[2 + s], 0);
+        if (e == hipSuccess && length_out) e = hipMemcpyAsync(out_len(s), dl.p, 4ull * len, hipMemcpyDeviceToHost, back);
+        if (e == hipSuccess && sync_out) e = hipMemcpyAsync(out_sync(s), ds.p, (size_t)len, hipMemcpyDeviceToHost, back);
+        if (e == hipSuccess && expanded_out)
+            e = hipMemcpyAsync(out_exp(s), de.p, 8ull * len, hipMemcpyDeviceToHost, back);
+        if (e == hipSuccess) e = hipEventRecord(pipe_ev[4 + s], back);
+        if (e == hipSuccess && k >= 1) rc = deliver(k - 1);
+        if (e != hipSuccess || rc) return drain(e != hipSuccess ? hip_fail(e, "pipelined chunk output") : rc);
+    }
+    rc = deliver(nchunks - 1);
+    hipError_t e = hipStreamSynchronize(stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(back);
+    join_all();
+    if (rc) return rc;
+    if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, ev0, ev1) == hipSuccess) last_ms = ms;  // the last chunk's kernels
+    int64_t viol = -1;
+    for (int64_t k = 0; k < nchunks; ++k) {
+        uint32_t ndef = 0;
+        unsigned long long v = ~0ull;
+        std::memcpy(&ndef, heads + 16 * k + kCtrDeferred, 4);
+        std::memcpy(&v, heads + 16 * k + kCtrViolation, 8);
+        if (ndef) return fail(SSW_ERR_INTERNAL, "an automaton of n=%d was deferred", n);
+        if (v != ~0ull && (viol < 0 || chunk_lo(k) + (int64_t)v < viol)) viol = chunk_lo(k) + (int64_t)v;
+    }
+    if (violation_out) *violation_out = viol;
+    return SSW_OK;
+}
+
 int ssw_context::compute_i32(int32_t n, int64_t count, const int32_t *A, const int32_t *B,
                              int32_t *length_out, uint8_t *sync_out, uint8_t *witness_out,
                              int32_t witness_stride, int64_t *expanded_out, int32_t bound,
@@ -1219,13 +1402,30 @@ int ssw_context::compute_i32(int32_t n, int64_t count, const int32_t *A, const i
     int rc = ensure_stage((size_t)count * 2 * n);
     if (rc) return rc;
     uint8_t *packed = (uint8_t *)h_stage;
-    // few large chunks: every solve pays its launches' tails (the longest
-    // searches of the chunk), so a chunk below ~2^18 automata costs more in
-    // tails than the packing it hides
     // Witness words make the tails far longer (the longest searches also run
-    // the witness passes), so those calls are solved whole.
-    constexpr int64_t kMinChunk = 1 << 18;
-    const int64_t nchunks = witness_out ? 1 : std::max<int64_t>(1, std::min<int64_t>(4, count / kMinChunk));
+    // the witness passes), so those calls are solved whole.  n <= 16: the
+    // chunks' transfers also overlap the kernels (host_pipeline).
+    const int64_t nchunks = witness_out ? 1 : std::max<int64_t>(1, std::min<int64_t>(4, count / kMinHostChunk));
+    if (nchunks > 1 && !witness_out && n <= kMaxLdsN) {
+        HIP_TRY(hipSetDevice(device), "hipSetDevice");
+        solve_begin((uint64_t)count);
+        host_no_length = length_out == nullptr;
+        rc = host_pipeline(n, count, nchunks,
+                           [&](int64_t lo, int64_t hi) { return pack_range(n, A, B, packed, lo, hi); },
+                           [&](int64_t first) {
+                               int q = 0;
+                               while (q < n && (uint32_t)A[first * n + q] < (uint32_t)n &&
+                                      (uint32_t)B[first * n + q] < (uint32_t)n)
+                                   ++q;
+                               return fail(SSW_ERR_ARG, "automaton %lld state %d: transition out of [0,%d)",
+                                           (long long)first, q, n);
+                           },
+                           length_out, sync_out, expanded_out, bound, violation_out);
+        host_no_length = false;
+        solve_end();
+        const int crc = checked_verdict();
+        return rc ? rc : crc;
+    }
     auto chunk_lo = [&](int64_t k) { return count * k / nchunks; };
     const int64_t hw = std::max<int64_t>(1, (int64_t)std::thread::hardware_concurrency());
     const int workers = (int)std::max<int64_t>(1, std::min<int64_t>(hw, 16) - (nchunks > 1 ? 1 : 0));
@@ -1273,127 +1473,6 @@ int ssw_context::compute_i32(int32_t n, int64_t count, const int32_t *A, const i
         while (q < n && (uint32_t)A[first * n + q] < (uint32_t)n && (uint32_t)B[first * n + q] < (uint32_t)n) ++q;
         return fail(SSW_ERR_ARG, "automaton %lld state %d: transition out of [0,%d)", (long long)first, q, n);
     };
-    if (nchunks > 1 && !witness_out && n <= kMaxLdsN) {
-        // Pipelined: chunk k's transitions are copied on the copy stream while
-        // chunk k - 1's launches run, into the device set k & 1; its results
-        // come back into pinned memory on a third stream once its launches
-        // are done and are copied to the caller while chunk k + 1 runs.  The host waits only for packed
-        // chunks and finished results; the solves' counters (n <= 16: no
-        // automaton is deferred) are read once at the end.
-        const uint64_t most = (uint64_t)((count + nchunks - 1) / nchunks);
-        const uint64_t out_row = 4u + (sync_out ? 1u : 0u) + (expanded_out ? 8u : 0u);
-        const uint64_t half = align_up(most * out_row + 16, 256);
-        if ((rc = ensure(d_delta2, most * 2u * (uint64_t)n, "delta"))) return rc;
-        if ((rc = ensure(d_len2, 4ull * most, "length"))) return rc;
-        if (sync_out && (rc = ensure(d_sync2, most, "sync"))) return rc;
-        if (expanded_out && (rc = ensure(d_exp2, 8ull * most, "expanded"))) return rc;
-        if ((rc = ensure_out(2 * half + 16u * (uint64_t)nchunks))) return rc;
-        uint8_t *heads = (uint8_t *)h_out + 2 * half;
-        // inputs and outputs on streams of their own, so that neither waits
-        // behind the other's dependencies
-        hipStream_t copy = class_streams[kClassStreams - 1], back = class_streams[kClassStreams - 2];
-        // host part of a set's results: lengths, then flags, then counts
-        const auto out_len = [&](int s) { return (int32_t *)((uint8_t *)h_out + s * half); };
-        const auto out_sync = [&](int s) { return (uint8_t *)(out_len(s) + most); };
-        const auto out_exp = [&](int s) {
-            return (int64_t *)((uint8_t *)h_out + s * half + align_up(most * (4u + (sync_out ? 1u : 0u)), 8));
-        };
-        const auto deliver = [&](int64_t k) -> int {  // chunk k's results to the caller
-            const int s = (int)(k & 1);
-            HIP_TRY(hipEventSynchronize(pipe_ev[4 + s]), "hipEventSynchronize");
-            const int64_t lo = chunk_lo(k), len = chunk_lo(k + 1) - lo;
-            if (length_out) std::memcpy(length_out + lo, out_len(s), 4ull * len);
-            if (sync_out) std::memcpy(sync_out + lo, out_sync(s), (size_t)len);
-            if (expanded_out) std::memcpy(expanded_out + lo, out_exp(s), 8ull * len);
-            return SSW_OK;
-        };
-        const auto drain = [&](int code) {
-            abort_pack.store(true);
-            join_all();
-            hipStreamSynchronize(copy);
-            hipStreamSynchronize(stream);
-            hipStreamSynchronize(back);
-            solve_end();
-            return code;
-        };
-        HIP_TRY(hipSetDevice(device), "hipSetDevice");
-        solve_begin((uint64_t)count);
-        host_no_length = length_out == nullptr;
-        for (int64_t k = 0; k < nchunks; ++k) {
-            while (packed_parts[k].load(std::memory_order_acquire) < workers) std::this_thread::yield();
-            if (bad[k].load() >= 0) {
-                const int64_t first = bad[k].load();
-                host_no_length = false;
-                drain(0);
-                return bad_input(first);
-            }
-            const int s = (int)(k & 1);
-            const int64_t lo = chunk_lo(k), len = chunk_lo(k + 1) - lo;
-            DevBuf &dd = s ? d_delta2 : d_delta, &dl = s ? d_len2 : d_len, &ds = s ? d_sync2 : d_sync,
-                   &de = s ? d_exp2 : d_exp;
-            hipError_t e = hipSuccess;
-            // the set's inputs are overwritten once chunk k - 2's launches are done with them
-            if (k >= 2) e = hipStreamWaitEvent(copy, pipe_ev[2 + s], 0);
-            if (e == hipSuccess)
-                e = hipMemcpyAsync(dd.p, packed + (size_t)lo * 2 * n, (size_t)len * 2 * n, hipMemcpyHostToDevice, copy);
-            if (e == hipSuccess) e = hipEventRecord(pipe_ev[s], copy);
-            if (e == hipSuccess) e = hipStreamWaitEvent(stream, pipe_ev[s], 0);
-            // the set's results are overwritten once chunk k - 2's have been copied back
-            if (e == hipSuccess && k >= 2) e = hipStreamWaitEvent(stream, pipe_ev[4 + s], 0);
-            if (e != hipSuccess) {
-                host_no_length = false;
-                return drain(hip_fail(e, "pipelined chunk input"));
-            }
-            DeviceBatch db;
-            db.count = len;
-            db.uniform_n = n;
-            db.bound = bound;
-            db.delta = (const uint8_t *)dd.p;
-            db.length = (int32_t *)dl.p;
-            db.sync = sync_out ? (uint8_t *)ds.p : nullptr;
-            db.expanded = expanded_out ? (int64_t *)de.p : nullptr;
-            rc = solve_device(db, nullptr, false, nullptr, nullptr, heads + 16 * k);
-            if (rc) {
-                host_no_length = false;
-                return drain(rc);
-            }
-            e = hipEventRecord(pipe_ev[2 + s], stream);
-            if (e == hipSuccess) e = hipStreamWaitEvent(back, pipe_ev[2 + s], 0);
-            if (e == hipSuccess && length_out)
-                e = hipMemcpyAsync(out_len(s), dl.p, 4ull * len, hipMemcpyDeviceToHost, back);
-            if (e == hipSuccess && sync_out) e = hipMemcpyAsync(out_sync(s), ds.p, (size_t)len, hipMemcpyDeviceToHost, back);
-            if (e == hipSuccess && expanded_out)
-                e = hipMemcpyAsync(out_exp(s), de.p, 8ull * len, hipMemcpyDeviceToHost, back);
-            if (e == hipSuccess) e = hipEventRecord(pipe_ev[4 + s], back);
-            if (e == hipSuccess && k >= 1) rc = deliver(k - 1);
-            if (e != hipSuccess || rc) {
-                host_no_length = false;
-                return drain(e != hipSuccess ? hip_fail(e, "pipelined chunk output") : rc);
-            }
-        }
-        host_no_length = false;
-        rc = deliver(nchunks - 1);
-        hipError_t e = hipStreamSynchronize(stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(back);
-        join_all();
-        solve_end();
-        if (rc) return rc;
-        if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
-        float ms = 0;
-        if (hipEventElapsedTime(&ms, ev0, ev1) == hipSuccess) last_ms = ms;  // the last chunk's kernels
-        for (int64_t k = 0; k < nchunks; ++k) {
-            uint32_t ndef = 0;
-            unsigned long long v = ~0ull;
-            std::memcpy(&ndef, heads + 16 * k + kCtrDeferred, 4);
-            std::memcpy(&v, heads + 16 * k + kCtrViolation, 8);
-            if (ndef) return fail(SSW_ERR_INTERNAL, "an automaton of n=%d was deferred", n);
-            if (v != ~0ull && (viol < 0 || chunk_lo(k) + (int64_t)v < viol)) viol = chunk_lo(k) + (int64_t)v;
-        }
-        const int crc = checked_verdict();
-        if (crc) return crc;
-        if (violation_out) *violation_out = viol;
-        return SSW_OK;
-    }
     for (int64_t k = 0; k < nchunks; ++k) {
         while (packed_parts[k].load(std::memory_order_acquire) < workers) std::this_thread::yield();
         const int64_t first = bad[k].load();

@@ -418,3 +418,32 @@ def test_host_arrays_pipelined_bad_input_late(ctx):
     r = ctx.compute(n, A[:1000], B[:1000])
     L, _, _, _ = O.solve(n, A[:1000], B[:1000], threads=8)
     np.testing.assert_array_equal(r.length, L)
+
+
+def test_host_rows_pipelined_n16(ctx):
+    """Packed rows in host memory through ssw_solve_batch in pipelined chunks
+    (rows copied into pinned staging and checked by host threads under the
+    earlier chunks' transfers and kernels) against the oracle, the violation
+    index with a lowered bound, and a bad row in the second chunk."""
+    n = 16
+    count = 560_000  # two chunks
+    A, B = ssw.generate_uniform(1619, n, count)
+    ca, cb = ssw.generate_cerny(n)
+    for i in (3, 279_999, 280_000, count - 2):
+        A[i], B[i] = ca, cb
+    rows = ssw.pack_uniform(A, B).reshape(-1)
+    print("pipelined rows n=16: oracle", flush=True)
+    L, S, E, _ = O.solve(n, A, B, threads=8)
+    print("pipelined rows n=16: solve", flush=True)
+    r = ctx.solve_packed(rows, count, n)
+    np.testing.assert_array_equal(r.length, L)
+    np.testing.assert_array_equal(r.sync, S)
+    np.testing.assert_array_equal(r.expanded, E)
+    assert r.violation == -1
+    assert ctx.solve_packed(rows, count, n, bound=100).violation == 3
+    bad = rows.copy().reshape(count, 2 * n)
+    bad[300_007, 2 * n - 1] = n
+    with pytest.raises(ssw.SswError, match="300007"):
+        ctx.solve_packed(bad.reshape(-1), count, n)
+    r2 = ctx.solve_packed(rows[: 2 * n * 1000], 1000, n)
+    np.testing.assert_array_equal(r2.length, L[:1000])
