@@ -229,8 +229,12 @@ struct ssw_context {
     // short; call_count = automata of the running solve
     std::atomic<uint64_t> solve_gen{0};
     std::atomic<uint64_t> call_count{0};
+    // automata of the running call already handed back by solves before the
+    // current one (a pipelined call runs several solves)
+    std::atomic<uint64_t> call_done_base{0};
     void solve_begin(uint64_t count) {
         call_count.store(count);
+        call_done_base.store(0);
         solve_gen.fetch_add(1);
     }
     void solve_end() { solve_gen.fetch_add(1); }

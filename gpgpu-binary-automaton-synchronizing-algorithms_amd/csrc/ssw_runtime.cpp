@@ -1316,6 +1316,7 @@ int ssw_context::host_pipeline(int32_t n, int64_t count, int64_t nchunks,
         if (length_out) std::memcpy(length_out + lo, out_len(s), 4ull * len);
         if (sync_out) std::memcpy(sync_out + lo, out_sync(s), (size_t)len);
         if (expanded_out) std::memcpy(expanded_out + lo, out_exp(s), 8ull * len);
+        call_done_base.fetch_add((uint64_t)len);  // the progress feed's in-flight count
         return SSW_OK;
     };
     const auto drain = [&](int code) {
@@ -1569,7 +1570,7 @@ int ssw_context::read_progress(ssw_progress *out) {
     // a solve running from the first read to the second: its count is the
     // one the done-in-this-solve counter belongs to
     const uint64_t g0 = solve_gen.load();
-    const uint64_t cnt = call_count.load();
+    const uint64_t cnt = call_count.load(), base = call_done_base.load();
     std::vector<unsigned long long> raw(kProgressBytes / 8);
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) e = hipMemcpyAsync(raw.data(), prog.p, kProgressBytes, hipMemcpyDeviceToHost, monitor);
@@ -1583,7 +1584,7 @@ int ssw_context::read_progress(ssw_progress *out) {
     out->subsets_expanded = v[1] + large_subsets.load();
     out->launches = total_launches.load();
     out->deferred = deferred_total;
-    out->in_flight = running && cnt > v[2] ? cnt - v[2] : 0;
+    out->in_flight = running && cnt > base + v[2] ? cnt - base - v[2] : 0;
     return SSW_OK;
 }
 

@@ -40,6 +40,28 @@ def test_progress_counters_and_callback(ctx):
     assert all(0 <= p["in_flight"] <= len(delta) for p in seen)
 
 
+def test_progress_of_a_pipelined_host_call(ctx):
+    # int32 host arrays of 2^19+ automata run as several solves in one call:
+    # in flight counts the call's automata not handed back yet, never more
+    # than the call, and is 0 once the call returned
+    seen = []
+    n, delta, pos = W.uniform_n16(count=600_000)
+    A = np.ascontiguousarray(delta[:, :n]).astype(np.int32)
+    B = np.ascontiguousarray(delta[:, n:]).astype(np.int32)
+    before = ctx.progress()
+    ctx.set_progress_callback(seen.append, 1)
+    ctx.compute(n, A, B)
+    after = ctx.progress()
+    time.sleep(0.02)
+    ctx.set_progress_callback(None)
+    assert after["automata_done"] - before["automata_done"] == len(delta)
+    assert after["in_flight"] == 0
+    assert seen and seen[-1]["in_flight"] == 0
+    assert all(0 <= p["in_flight"] <= len(delta) for p in seen)
+    done = [p["automata_done"] for p in seen]
+    assert done == sorted(done)
+
+
 def test_progress_reports_a_solve_shorter_than_the_interval(ctx):
     # a solve that starts and ends between two looks of the library's thread
     # (interval 1 s, a few hundred microseconds of work) is still reported:
