@@ -449,7 +449,7 @@ __global__ void init_queues_kernel(uint32_t *q0, uint32_t *q1, uint32_t *q2, int
 
 // The per-solve counters: deferred count <- 0, smallest violating index <-
 // none (either may be null), automata done in this solve <- 0 in every copy
-// of the progress counters, the launch statistics <- 0.
+// of the progress counters (unless null), the launch statistics <- 0.
 __global__ void reset_counters_kernel(uint32_t *deferred, unsigned long long *violation,
                                       unsigned long long *progress, unsigned long long *launch_stats,
                                       uint32_t stat_words) {
@@ -457,7 +457,8 @@ __global__ void reset_counters_kernel(uint32_t *deferred, unsigned long long *vi
         if (deferred) *deferred = 0u;
         if (violation) *violation = ~0ull;
     }
-    for (uint32_t k = threadIdx.x; k < kCounterSlots; k += blockDim.x) progress[(size_t)k * kSlotWords + 2] = 0ull;
+    if (progress)
+        for (uint32_t k = threadIdx.x; k < kCounterSlots; k += blockDim.x) progress[(size_t)k * kSlotWords + 2] = 0ull;
     for (uint32_t k = threadIdx.x; k < stat_words; k += blockDim.x) launch_stats[k] = 0ull;
 }
 
@@ -1237,7 +1238,7 @@ struct SmallSolver {
             const uint32_t ib = g.or_low32(r < (uint32_t)n ? 1u << qb : 0u);
             const uint32_t I = ia & ib;
             const uint32_t ka = (uint32_t)__popc(ia), kb = (uint32_t)__popc(ib), ki = (uint32_t)__popc(I);
-            if (ka > 15u || kb > 15u) return spill_to(P.spill_queue, P.spill_list, 0u);
+            if (ka > 15u || kb > 15u) return spill_to(P.spill_queue, P.spill_list, 0u, P.spill_base);
             // label of state x in the family of `fam` (a(Q) or b(Q)): a(Q) & b(Q) first
             const auto lab = [I, ki](uint32_t fam, uint32_t x) {
                 const uint32_t below = (1u << x) - 1u;
@@ -1279,9 +1280,9 @@ struct SmallSolver {
                 if (P.skip_list && k == (uint32_t)n) {
                     const uint32_t ia = g.or_low32(r < (uint32_t)n ? 1u << qa : 0u);
                     const uint32_t ib = g.or_low32(r < (uint32_t)n ? 1u << qb : 0u);
-                    if (ia == Q || ib == Q) return spill_to(P.skip_queue, P.skip_list, 0u);
+                    if (ia == Q || ib == Q) return spill_to(P.skip_queue, P.skip_list, 0u, P.skip_base);
                 }
-                return spill_to(P.spill_queue, P.spill_list, 0u);
+                return spill_to(P.spill_queue, P.spill_list, 0u, P.spill_base);
             }
             const auto label = [U, k](uint32_t x) {
                 const uint32_t below = (uint32_t)__popc(U & ((1u << x) - 1u));
@@ -1608,11 +1609,13 @@ struct SmallSolver {
 
     // Hands the automaton to the next launch (larger LDS ring), which
     // searches it again; `partial` = subsets expanded here so far.
-    __device__ int spill() { return spill_to(P.spill_queue, P.spill_list, rd + e0); }
-    __device__ int spill_to(uint32_t *queue, uint32_t *list, uint32_t partial) {
+    __device__ int spill() { return spill_to(P.spill_queue, P.spill_list, rd + e0, P.spill_base); }
+    // list entries are the automaton's index plus `base` (a list shared by
+    // several batches that are parts of one: their place in the whole)
+    __device__ int spill_to(uint32_t *queue, uint32_t *list, uint32_t partial, uint32_t base) {
         if (g.rank() == 0) {
             const uint32_t k = atomicAdd(queue + seg * kQueueWords + 1, 1u);
-            list[k] = (uint32_t)ai;
+            list[k] = (uint32_t)ai + base;
         }
         expanded_local += partial;
         return kSpilled;

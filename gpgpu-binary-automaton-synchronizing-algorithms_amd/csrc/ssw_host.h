@@ -64,6 +64,13 @@ struct ClassSpec {
     // from this caller-provided memory and every earlier launch spills
     uint8_t *static_slots = nullptr;
     uint64_t static_slot_bytes = 0;
+    // one part of a larger batch: the spill chain stops before its last
+    // launch, which the caller runs once for all parts (solve_last_launch)
+    // over spill_queue[1] / spill_list[1]; the parts share that queue and
+    // list (set up by the caller) and list their automata as the part's
+    // indices plus list1_base
+    bool defer_last = false;
+    uint32_t list1_base = 0;
 };
 
 // launch passes of one class
@@ -85,6 +92,8 @@ struct PassIO {
     int general_entries = 1;            // frontier entries per lane of the general solver (1 or 4)
     bool general_fam = false;           // ... with the two-family encoding (n > 16, 4 entries per lane)
     bool queue_ready = false;           // the class's queue was set up with the spill queues
+    uint32_t skip_base = 0;             // added to the indices written to the skip list ...
+    uint32_t out_base = 0;              // ... and to the out list
 };
 constexpr int kClassQueues = 3;  // work queues per launch class: its own and two spill queues
 
@@ -208,6 +217,8 @@ struct ssw_context {
     ssw::DevBuf d_delta, d_nper, d_off, d_len, d_sync, d_exp, d_wit;
     // the second buffer set of the host pipeline (chunk k uses set k & 1)
     ssw::DevBuf d_delta2, d_len2, d_sync2, d_exp2;
+    // with the last launch run once for all chunks: its work queue and list
+    ssw::DevBuf pipe_q, pipe_list;
     // its streams' events: inputs of set s copied (0, 1), set s's kernels
     // done with them (2, 3), outputs of set s copied back (4, 5)
     hipEvent_t pipe_ev[6] = {};
@@ -267,7 +278,13 @@ struct ssw_context {
     // batch of n <= 16, whose automata are never deferred; the caller reads
     // the violation once the stream is drained)
     int solve_device(const ssw::DeviceBatch &db, const uint8_t *host_n, bool plain, int64_t *violation_out,
-                     const int64_t *size_hist = nullptr, void *head_out = nullptr);
+                     const int64_t *size_hist = nullptr, void *head_out = nullptr,
+                     const ssw::ClassSpec *defer = nullptr);
+    // the last launch of the parts of a uniform batch of n <= 16 whose
+    // chains stopped before it (ClassSpec::defer_last), over their shared
+    // list, on the whole batch; its counters copied to head_out
+    int solve_last_launch(const ssw::DeviceBatch &db, uint32_t *queue1, uint32_t *list1, int64_t items,
+                          void *head_out);
     int solve_host_packed(const ssw_batch *b, int mode, int64_t *violation_out);
     // a uniform batch of n <= 16 from host memory in pipelined chunks (fill
     // packs automata [lo, hi) into the pinned staging and returns the first

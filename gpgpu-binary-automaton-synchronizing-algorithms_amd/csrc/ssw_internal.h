@@ -145,6 +145,10 @@ struct BatchParams {
     // their first levels
     uint32_t *skip_queue;
     uint32_t *skip_list;
+    // added to the automaton indices written to the skip list and to the
+    // spill list (lists shared by the parts of a larger batch)
+    uint32_t skip_base;
+    uint32_t spill_base;
     // n <= 16 path: > 0 relabels every automaton's states so that
     // U = a(Q) | b(Q) takes the lowest labels and uses a visited bitmap of
     // 2^narrow_bits bits; automata with |U| > narrow_bits are spilled at start

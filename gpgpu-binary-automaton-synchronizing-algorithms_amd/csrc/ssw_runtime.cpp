@@ -144,7 +144,7 @@ void ssw_context::close() {
     stop_progress_thread();
     if (device >= 0) hipSetDevice(device);
     for (DevBuf *b : {&ctrs, &queues, &deferred, &order, &spill, &lstats, &prog, &sizes, &d_delta, &d_nper, &d_off, &d_len,
-                      &d_sync, &d_exp, &d_wit, &d_delta2, &d_len2, &d_sync2, &d_exp2})
+                      &d_sync, &d_exp, &d_wit, &d_delta2, &d_len2, &d_sync2, &d_exp2, &pipe_q, &pipe_list})
         if (b->p) hipFree(b->p);
     for (int k = 0; k < 2 * kMaxLaunchRecords; ++k)
         if (lev[k]) hipEventDestroy(lev[k]);
@@ -303,6 +303,33 @@ int ssw_context::launch_class(const DeviceBatch &db, const ClassSpec &cls, bool 
         return launch_pass(db, cls, t, plain, PassIO{});
     const int nb = (t.narrow_bits >= 8 && t.narrow_bits < cls.nmax) ? t.narrow_bits : 0;
     hipStream_t st = cls.stream ? cls.stream : stream;
+    if (cls.defer_last) {
+        // the first two launches only; what the last one takes goes to the
+        // parts' shared list
+        if (!nb) return fail(SSW_ERR_INTERNAL, "deferred last launch without the narrow chain (n=%d)", cls.nmax);
+        HIP_TRY(init_queues3(cls.queues, cls.spill_queue[0], nullptr, cls.items, 2u, st), "init queues");
+        PassIO first;
+        first.queue_ready = true;
+        first.kind = kPassSpillFrom;
+        first.out_list = cls.spill_list[0];
+        first.out_queue = cls.spill_queue[0];
+        first.narrow_bits = nb;
+        first.skip_list = cls.spill_list[1];
+        first.skip_queue = cls.spill_queue[1];
+        first.skip_base = cls.list1_base;
+        int rc = launch_pass(db, cls, t, false, first);
+        if (rc) return rc;
+        PassIO mid;
+        mid.kind = kPassSpillFrom;
+        mid.in_list = cls.spill_list[0];
+        mid.in_queue = cls.spill_queue[0];
+        mid.out_list = cls.spill_list[1];
+        mid.out_queue = cls.spill_queue[1];
+        mid.out_base = cls.list1_base;
+        ssw_tuning tm = t;
+        tm.chunk = std::min(t.chunk, 4);
+        return launch_pass(db, cls, tm, false, mid);
+    }
     // one launch sets up the class's queue (full) and the later launches'
     // (the first launch's segments, empty)
     HIP_TRY(init_queues3(cls.queues, cls.spill_queue[0], nb ? cls.spill_queue[1] : nullptr, cls.items, 6u, st),
@@ -456,6 +483,8 @@ int ssw_context::launch_pass(const DeviceBatch &db, const ClassSpec &cls, const 
     P.spill_list = io.out_list;
     P.skip_queue = io.skip_queue;
     P.skip_list = io.skip_list;
+    P.skip_base = io.skip_base;
+    P.spill_base = io.out_base;
     P.narrow_bits = small ? io.narrow_bits : 0;
     // only Verify reads nothing but the violation, so only it may skip searches
     P.prefilter = (host_no_length && verify_prefilter) ? 1u : 0u;
@@ -663,7 +692,8 @@ int ssw_context::launch_pass(const DeviceBatch &db, const ClassSpec &cls, const 
 
 // Splits the batch into launch classes: (thread-group width, bitmap in LDS).
 int ssw_context::solve_device(const DeviceBatch &db, const uint8_t *host_n, bool plain,
-                              int64_t *violation_out, const int64_t *size_hist, void *head_out) {
+                              int64_t *violation_out, const int64_t *size_hist, void *head_out,
+                              const ClassSpec *defer) {
     // the sizes one by one, copied from the device when a route needs them
     std::vector<uint8_t> fetched_n;
     auto need_host_n = [&]() -> int {
@@ -831,6 +861,14 @@ int ssw_context::solve_device(const DeviceBatch &db, const uint8_t *host_n, bool
                 cls.spill_queue[j] = (uint32_t *)((uint8_t *)cls.queues + kQueueBytes * (j + 1));
                 cls.spill_list[j] = (uint32_t *)spill.p + j * db.count + class_start[k];
             }
+            if (defer && single_class && db.uniform_n > 0) {
+                // a part of a larger batch whose last launch runs once for
+                // all parts (solve_last_launch)
+                cls.defer_last = true;
+                cls.spill_queue[1] = defer->spill_queue[1];
+                cls.spill_list[1] = defer->spill_list[1];
+                cls.list1_base = defer->list1_base;
+            }
             rc = launch_class(db, cls, false);
             if (rc) return rc;
         }
@@ -873,6 +911,50 @@ int ssw_context::solve_device(const DeviceBatch &db, const uint8_t *host_n, bool
         HIP_TRY(hipStreamSynchronize(stream), "hipStreamSynchronize");
     }
     if (violation_out) *violation_out = viol == ~0ull ? -1 : (int64_t)viol;
+    return SSW_OK;
+}
+
+// The last launch of a uniform batch of n <= 16 solved in parts whose
+// chains stopped before it: it searches the automata listed in list1
+// (indices into the whole batch, appended by the parts' launches through
+// queue1) with the larger ring and byte tables, on the context's stream; the
+// 16 counter bytes (deferred count, violation) are copied to head_out
+// without waiting.
+int ssw_context::solve_last_launch(const DeviceBatch &db, uint32_t *queue1, uint32_t *list1, int64_t items,
+                                   void *head_out) {
+    const int n = db.uniform_n;
+    if (n < 1 || n > kMaxLdsN) return fail(SSW_ERR_INTERNAL, "last launch of n=%d on its own", n);
+    launches = 0;
+    ++call_gen;
+    records.clear();
+    HIP_TRY(reset_counters((uint32_t *)((uint8_t *)ctrs.p + kCtrDeferred),
+                           (unsigned long long *)((uint8_t *)ctrs.p + kCtrViolation),
+                           (unsigned long long *)prog.p, (unsigned long long *)lstats.p, kLaunchStatWords, stream),
+            "reset counters");
+    HIP_TRY(hipEventRecord(ev0, stream), "hipEventRecord");
+    ClassSpec cls;
+    cls.nmax = n;
+    cls.lds = true;
+    cls.items = items;
+    cls.stream = stream;
+    cls.queues = queue1;
+    const ssw_tuning &t = tune[n];
+    ssw_tuning t2 = t;
+    if (t.spill_group_lanes > 0) t2.group_lanes = t.spill_group_lanes;
+    t2.block_threads = t2.group_lanes > 64 ? t2.group_lanes : 0;
+    t2.chunk = 1;
+    t2.arena_lds = t.spill_arena_lds;
+    t2.lane_entries = t.spill_lane_entries > 0 ? t.spill_lane_entries : t.lane_entries;
+    PassIO last;
+    last.kind = kPassSpillTo;
+    last.in_list = list1;
+    last.in_queue = queue1;
+    last.byte_tab = true;
+    const int rc = launch_pass(db, cls, t2, false, last);
+    if (rc) return rc;
+    HIP_TRY(hipEventRecord(ev1, stream), "hipEventRecord");
+    HIP_TRY(hipMemcpyAsync(head_out, ctrs.p, 16, hipMemcpyDeviceToHost, stream), "D2H");
+    last_launches = launches;
     return SSW_OK;
 }
 
@@ -1248,6 +1330,11 @@ static int64_t pack_range(int32_t n, const int32_t *A, const int32_t *B, uint8_t
 // for filled chunks and finished results; the solves' counters (n <= 16: no
 // automaton is deferred) are read once at the end.  An invalid automaton
 // drains the work already queued and returns bad_at(index).
+// With the three-launch chain (n = 16) the chunks' rows go to their place in
+// one whole-batch device buffer and each chunk runs only the first two
+// launches, appending what the last launch takes to one shared list; that
+// launch runs once after the last chunk and the results come back in one
+// copy: the chunked chains each ended in the tail of their longest searches.
 // (Solving the whole batch once after pipelined copies instead was measured:
 // the 1M n = 16 batch gains little, since its host packing is as long as the
 // copies, while the n = 13 Verify workload loses the overlap of its packing
@@ -1267,20 +1354,45 @@ int ssw_context::host_pipeline(int32_t n, int64_t count, int64_t nchunks,
         filled[k].store(0);
         bad[k].store(-1);
     }
-    const uint64_t most = (uint64_t)((count + nchunks - 1) / nchunks);
+    // n with the three-launch chain: every chunk runs the first two launches
+    // as it arrives, and the last launch (the longest searches, whose tails
+    // would otherwise end every chunk) runs once over what the chunks hand
+    // on; the chunks' rows and results have their place in one set of
+    // whole-batch buffers
+    const ssw_tuning &tn = tune[n];
+    const bool defer = tn.narrow_bits >= 8 && tn.narrow_bits < n && tn.spill_arena_lds > 0;
+    const uint64_t most = defer ? (uint64_t)count : (uint64_t)((count + nchunks - 1) / nchunks);
     const uint64_t out_row = 4u + (sync_out ? 1u : 0u) + (expanded_out ? 8u : 0u);
     const uint64_t half = align_up(most * out_row + 16, 256);
+    // the shared list of the last launch: its 64 queue segments each hold
+    // nchunks + 1 more entries than a 64th of the batch, so the chunks'
+    // appends (at most ceil(chunk / 64) per segment and chunk) never run
+    // into the next segment
+    const int64_t list_items = count + 64 * (nchunks + 1);
     int rc;
-    if ((rc = ensure(d_delta, most * 2u * (uint64_t)n, "delta"))) return rc;
-    if ((rc = ensure(d_len, 4ull * most, "length"))) return rc;
-    if (sync_out && (rc = ensure(d_sync, most, "sync"))) return rc;
-    if (expanded_out && (rc = ensure(d_exp, 8ull * most, "expanded"))) return rc;
-    if ((rc = ensure(d_delta2, most * 2u * (uint64_t)n, "delta"))) return rc;
-    if ((rc = ensure(d_len2, 4ull * most, "length"))) return rc;
-    if (sync_out && (rc = ensure(d_sync2, most, "sync"))) return rc;
-    if (expanded_out && (rc = ensure(d_exp2, 8ull * most, "expanded"))) return rc;
-    if ((rc = ensure_out(2 * half + 16u * (uint64_t)nchunks))) return rc;
-    uint8_t *heads = (uint8_t *)h_out + 2 * half;
+    if (defer) {
+        if ((rc = ensure(d_delta, (uint64_t)count * 2u * (uint64_t)n, "delta"))) return rc;
+        if ((rc = ensure(d_len, 4ull * count, "length"))) return rc;
+        if (sync_out && (rc = ensure(d_sync, (uint64_t)count, "sync"))) return rc;
+        if (expanded_out && (rc = ensure(d_exp, 8ull * count, "expanded"))) return rc;
+        if ((rc = ensure(deferred, 4ull * count, "deferred list"))) return rc;
+        if ((rc = ensure(pipe_q, kQueueBytes, "work queue"))) return rc;
+        if ((rc = ensure(pipe_list, 4ull * list_items, "spill list"))) return rc;
+        if ((rc = ensure_out(half + 16u * (uint64_t)(nchunks + 1)))) return rc;
+        // set up before the first chunk's launches, on their stream
+        HIP_TRY(init_queues((uint32_t *)pipe_q.p, list_items, stream, true), "init queues");
+    } else {
+        if ((rc = ensure(d_delta, most * 2u * (uint64_t)n, "delta"))) return rc;
+        if ((rc = ensure(d_len, 4ull * most, "length"))) return rc;
+        if (sync_out && (rc = ensure(d_sync, most, "sync"))) return rc;
+        if (expanded_out && (rc = ensure(d_exp, 8ull * most, "expanded"))) return rc;
+        if ((rc = ensure(d_delta2, most * 2u * (uint64_t)n, "delta"))) return rc;
+        if ((rc = ensure(d_len2, 4ull * most, "length"))) return rc;
+        if (sync_out && (rc = ensure(d_sync2, most, "sync"))) return rc;
+        if (expanded_out && (rc = ensure(d_exp2, 8ull * most, "expanded"))) return rc;
+        if ((rc = ensure_out(2 * half + 16u * (uint64_t)nchunks))) return rc;
+    }
+    uint8_t *heads = (uint8_t *)h_out + (defer ? 1 : 2) * half;
     // inputs and outputs on streams of their own, so that neither waits
     // behind the other's dependencies
     hipStream_t copy = class_streams[kClassStreams - 1], back = class_streams[kClassStreams - 2];
@@ -1327,6 +1439,80 @@ int ssw_context::host_pipeline(int32_t n, int64_t count, int64_t nchunks,
         hipStreamSynchronize(back);
         return code;
     };
+    // the smallest violating index of the solves whose counters are at heads
+    // (part k's indices start at lo(k)); no automaton of n <= 16 is deferred
+    const auto read_heads = [&](int64_t parts, const std::function<int64_t(int64_t)> &lo) -> int {
+        int64_t viol = -1;
+        for (int64_t k = 0; k < parts; ++k) {
+            uint32_t ndef = 0;
+            unsigned long long v = ~0ull;
+            std::memcpy(&ndef, heads + 16 * k + kCtrDeferred, 4);
+            std::memcpy(&v, heads + 16 * k + kCtrViolation, 8);
+            if (ndef) return fail(SSW_ERR_INTERNAL, "an automaton of n=%d was deferred", n);
+            if (v != ~0ull && (viol < 0 || lo(k) + (int64_t)v < viol)) viol = lo(k) + (int64_t)v;
+        }
+        if (violation_out) *violation_out = viol;
+        return SSW_OK;
+    };
+    if (defer) {
+        ClassSpec part;
+        part.spill_queue[1] = (uint32_t *)pipe_q.p;
+        part.spill_list[1] = (uint32_t *)pipe_list.p;
+        for (int64_t k = 0; k < nchunks; ++k) {
+            while (filled[k].load(std::memory_order_acquire) < workers) std::this_thread::yield();
+            if (bad[k].load() >= 0) {
+                const int64_t first = bad[k].load();
+                drain(0);
+                return bad_at(first);
+            }
+            const int s = (int)(k & 1);
+            const int64_t lo = chunk_lo(k), len = chunk_lo(k + 1) - lo;
+            hipError_t e = hipMemcpyAsync((uint8_t *)d_delta.p + (size_t)lo * 2 * n, packed + (size_t)lo * 2 * n,
+                                          (size_t)len * 2 * n, hipMemcpyHostToDevice, copy);
+            if (e == hipSuccess) e = hipEventRecord(pipe_ev[s], copy);
+            if (e == hipSuccess) e = hipStreamWaitEvent(stream, pipe_ev[s], 0);
+            if (e != hipSuccess) return drain(hip_fail(e, "pipelined chunk input"));
+            DeviceBatch db;
+            db.count = len;
+            db.uniform_n = n;
+            db.bound = bound;
+            db.delta = (const uint8_t *)d_delta.p + (size_t)lo * 2 * n;
+            db.length = (int32_t *)d_len.p + lo;
+            db.sync = sync_out ? (uint8_t *)d_sync.p + lo : nullptr;
+            db.expanded = expanded_out ? (int64_t *)d_exp.p + lo : nullptr;
+            part.list1_base = (uint32_t)lo;
+            rc = solve_device(db, nullptr, false, nullptr, nullptr, heads + 16 * k, &part);
+            if (rc) return drain(rc);
+        }
+        join_all();
+        DeviceBatch whole;
+        whole.count = count;
+        whole.uniform_n = n;
+        whole.bound = bound;
+        whole.delta = (const uint8_t *)d_delta.p;
+        whole.length = (int32_t *)d_len.p;
+        whole.sync = sync_out ? (uint8_t *)d_sync.p : nullptr;
+        whole.expanded = expanded_out ? (int64_t *)d_exp.p : nullptr;
+        rc = solve_last_launch(whole, (uint32_t *)pipe_q.p, (uint32_t *)pipe_list.p, count, heads + 16 * nchunks);
+        if (rc) return drain(rc);
+        hipError_t e = hipSuccess;
+        if (length_out) e = hipMemcpyAsync(out_len(0), d_len.p, 4ull * count, hipMemcpyDeviceToHost, stream);
+        if (e == hipSuccess && sync_out)
+            e = hipMemcpyAsync(out_sync(0), d_sync.p, (size_t)count, hipMemcpyDeviceToHost, stream);
+        if (e == hipSuccess && expanded_out)
+            e = hipMemcpyAsync(out_exp(0), d_exp.p, 8ull * count, hipMemcpyDeviceToHost, stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        if (e != hipSuccess) return drain(hip_fail(e, "pipelined results"));
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, ev0, ev1) == hipSuccess) last_ms = ms;  // the last launch
+        parallel_ranges(count, 1 << 16, [&](int64_t lo, int64_t hi) {
+            if (length_out) std::memcpy(length_out + lo, out_len(0) + lo, 4ull * (hi - lo));
+            if (sync_out) std::memcpy(sync_out + lo, out_sync(0) + lo, (size_t)(hi - lo));
+            if (expanded_out) std::memcpy(expanded_out + lo, out_exp(0) + lo, 8ull * (hi - lo));
+        });
+        call_done_base.fetch_add((uint64_t)count);  // the progress feed's in-flight count
+        return read_heads(nchunks + 1, [&](int64_t k) { return k < nchunks ? chunk_lo(k) : (int64_t)0; });
+    }
     for (int64_t k = 0; k < nchunks; ++k) {
         while (filled[k].load(std::memory_order_acquire) < workers) std::this_thread::yield();
         if (bad[k].load() >= 0) {
@@ -1376,17 +1562,7 @@ int ssw_context::host_pipeline(int32_t n, int64_t count, int64_t nchunks,
     if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
     float ms = 0;
     if (hipEventElapsedTime(&ms, ev0, ev1) == hipSuccess) last_ms = ms;  // the last chunk's kernels
-    int64_t viol = -1;
-    for (int64_t k = 0; k < nchunks; ++k) {
-        uint32_t ndef = 0;
-        unsigned long long v = ~0ull;
-        std::memcpy(&ndef, heads + 16 * k + kCtrDeferred, 4);
-        std::memcpy(&v, heads + 16 * k + kCtrViolation, 8);
-        if (ndef) return fail(SSW_ERR_INTERNAL, "an automaton of n=%d was deferred", n);
-        if (v != ~0ull && (viol < 0 || chunk_lo(k) + (int64_t)v < viol)) viol = chunk_lo(k) + (int64_t)v;
-    }
-    if (violation_out) *violation_out = viol;
-    return SSW_OK;
+    return read_heads(nchunks, chunk_lo);
 }
 
 int ssw_context::compute_i32(int32_t n, int64_t count, const int32_t *A, const int32_t *B,
