@@ -404,6 +404,43 @@ def test_host_arrays_pipelined_n16(ctx):
     assert ctx.compute(n, A2, B2, bound=bound).violation == expected
 
 
+def test_host_pipeline_every_automaton_to_the_last_launch(ctx):
+    """The n = 16 host pipeline hands what the chunks' last launch takes to
+    one list shared by the chunks (each segment of its queue sized for a 64th
+    of the batch plus a slack of a few entries per chunk).  Here every
+    automaton has a permutation letter, so the first launch hands each one to
+    that list and the list is full to its last slot: results against the
+    oracle through both host entry points, and the violation index of an
+    automaton in the second chunk found by the last launch."""
+    n = 16
+    count = (1 << 19) + 12_345  # two chunks
+    rng = np.random.default_rng(1620)
+    A = np.argsort(rng.random((count, n)), axis=1).astype(np.int32)  # a permutes the states
+    B = rng.integers(0, n, size=(count, n), dtype=np.int32)
+    ca, cb = ssw.generate_cerny(n)
+    for i in (0, 300_001, count - 1):
+        A[i], B[i] = ca, cb
+    print("pipeline all deferred: oracle", flush=True)
+    L, S, E, _ = O.solve(n, A, B, threads=8)
+    print("pipeline all deferred: compute", flush=True)
+    r = ctx.compute(n, A, B)
+    np.testing.assert_array_equal(r.length, L)
+    np.testing.assert_array_equal(r.sync, S)
+    np.testing.assert_array_equal(r.expanded, E)
+    print("pipeline all deferred: packed rows", flush=True)
+    rows = ssw.pack_uniform(A, B).reshape(-1)
+    r2 = ctx.solve_packed(rows, count, n)
+    np.testing.assert_array_equal(r2.length, L)
+    np.testing.assert_array_equal(r2.expanded, E)
+    # a bound below C_16's 225 that every other automaton keeps: the first
+    # violation is C_16 at 0; without it, the one at 300,001 (second chunk)
+    bound = int(L[L != 225].max())
+    assert bound < 225
+    assert ctx.compute(n, A, B, bound=bound).violation == 0
+    A[0], B[0] = A[1], B[1]
+    assert ctx.verify(n, A, B, bound=bound) == 300_001
+
+
 def test_host_arrays_pipelined_bad_input_late(ctx):
     """A transition out of range in the second chunk, found while the first
     chunk is already on the device: the call fails naming that automaton,
