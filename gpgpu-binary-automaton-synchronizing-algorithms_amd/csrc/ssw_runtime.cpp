@@ -1571,6 +1571,7 @@ int ssw_context::compute_i32(int32_t n, int64_t count, const int32_t *A, const i
                              int64_t *violation_out) {
     if (n < 1 || n > SSW_MAX_N) return fail(SSW_ERR_ARG, "n=%d out of [1,30]", n);
     if (count < 0 || (count > 0 && (!A || !B))) return fail(SSW_ERR_ARG, "null argument");
+    if (count > (int64_t)0x7FFFFFFF) return fail(SSW_ERR_ARG, "count above 2^31-1");
     if (violation_out) *violation_out = -1;
     // flatten like SlimGPUQueue.cs:108-126, packed to bytes in pinned staging
     // memory by host threads, checked on the way.  Large calls are cut into
