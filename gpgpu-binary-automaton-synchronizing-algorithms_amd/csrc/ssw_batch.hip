@@ -462,6 +462,43 @@ __global__ void reset_counters_kernel(uint32_t *deferred, unsigned long long *vi
     for (uint32_t k = threadIdx.x; k < stat_words; k += blockDim.x) launch_stats[k] = 0ull;
 }
 
+// The results of the automata listed in a drained work queue's list (entries
+// [start of segment k, end of segment k) of a queue set up empty over
+// `items` slots), as 16-byte records {index | length << 32, expanded | flag
+// << 56} appended after out[0..1] (out[0] counts them, zeroed by the caller):
+// the host pipeline copies these back after the last launch of a batch whose
+// other results it copied earlier.
+__global__ void gather_listed_kernel(const uint32_t *queue, const uint32_t *list, uint32_t items,
+                                     const int32_t *length, const uint8_t *sync, const int64_t *expanded,
+                                     unsigned long long *out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool have = false;
+    uint32_t a = 0;
+    if (i < items) {
+        // segment k holds slots [items * k / kQueues, items * (k + 1) / kQueues)
+        uint32_t k = (uint32_t)(((uint64_t)i * kQueues) / items);
+        while (k + 1u < (uint32_t)kQueues && (uint64_t)items * (k + 1u) / kQueues <= i) ++k;
+        while (k > 0u && (uint64_t)items * k / kQueues > i) --k;
+        if (i < queue[k * kQueueWords + 1]) {
+            have = true;
+            a = list[i];
+        }
+    }
+    const uint64_t m = __ballot(have);
+    if (m == 0) return;
+    const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+    uint32_t base = 0;
+    if (have && below == 0u) base = atomicAdd((unsigned int *)out, (unsigned int)__popcll(m));
+    base = __builtin_amdgcn_readlane(base, (int)__builtin_ctzll(m));
+    if (have) {
+        const uint64_t pos = 2u + 2ull * (base + below);
+        const uint64_t e = expanded ? (uint64_t)expanded[a] : 0ull;
+        const uint64_t f = sync ? (uint64_t)sync[a] : 0ull;
+        out[pos] = (uint64_t)a | ((uint64_t)(uint32_t)length[a] << 32);
+        out[pos + 1] = (e & 0x00FFFFFFFFFFFFFFull) | (f << 56);
+    }
+}
+
 // The general solver (any n <= 30, HBM bitmaps past the LDS size, witness
 // words for n > 16): every lane takes T frontier entries per level step, the
 // test-and-sets of all of them in flight together (T > 1: the long searches
@@ -2601,6 +2638,14 @@ hipError_t reset_counters(uint32_t *deferred, unsigned long long *violation, uns
                           unsigned long long *launch_stats, uint32_t stat_words, hipStream_t st) {
     hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(256), 0, st, deferred, violation, progress,
                        launch_stats, stat_words);
+    return hipGetLastError();
+}
+
+hipError_t gather_listed(const uint32_t *queue, const uint32_t *list, uint32_t items, const int32_t *length,
+                         const uint8_t *sync, const int64_t *expanded, unsigned long long *out, hipStream_t st) {
+    if (items == 0) return hipSuccess;
+    hipLaunchKernelGGL(gather_listed_kernel, dim3((items + 255u) / 256u), dim3(256), 0, st, queue, list, items, length,
+                       sync, expanded, out);
     return hipGetLastError();
 }
 

@@ -217,11 +217,13 @@ struct ssw_context {
     ssw::DevBuf d_delta, d_nper, d_off, d_len, d_sync, d_exp, d_wit;
     // the second buffer set of the host pipeline (chunk k uses set k & 1)
     ssw::DevBuf d_delta2, d_len2, d_sync2, d_exp2;
-    // with the last launch run once for all chunks: its work queue and list
-    ssw::DevBuf pipe_q, pipe_list;
+    // with the last launch run once for all chunks: its work queue and list,
+    // and the records of the listed automata's results (gather_listed)
+    ssw::DevBuf pipe_q, pipe_list, pipe_patch;
     // its streams' events: inputs of set s copied (0, 1), set s's kernels
-    // done with them (2, 3), outputs of set s copied back (4, 5)
-    hipEvent_t pipe_ev[6] = {};
+    // done with them (2, 3), outputs of set s copied back (4, 5); with the
+    // last launch run once: chunk k's results copied back (4 + k, k < 4)
+    hipEvent_t pipe_ev[8] = {};
     std::map<ssw::SlotKey, ssw::DevBuf> slot_cache;
     std::map<ssw::SlotKey, uint64_t> slot_gen;   // call that last used each entry
     uint64_t call_gen = 0;

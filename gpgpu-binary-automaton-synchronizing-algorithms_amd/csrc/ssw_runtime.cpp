@@ -40,6 +40,8 @@ hipError_t init_queues3(uint32_t *q0, uint32_t *q1, uint32_t *q2, int64_t count,
                         hipStream_t st);
 hipError_t reset_counters(uint32_t *deferred, unsigned long long *violation, unsigned long long *progress,
                           unsigned long long *launch_stats, uint32_t stat_words, hipStream_t st);
+hipError_t gather_listed(const uint32_t *queue, const uint32_t *list, uint32_t items, const int32_t *length,
+                         const uint8_t *sync, const int64_t *expanded, unsigned long long *out, hipStream_t st);
 hipError_t noalloc_scratch(int device, void **out);
 std::mutex &noalloc_call_lock(int device);
 }  // namespace ssw
@@ -231,6 +233,8 @@ static inline uint64_t align_up(uint64_t x, uint64_t a) { return (x + a - 1) / a
 // launches' tails (the longest searches of the chunk), so a chunk below
 // ~2^18 automata costs more in tails than the host work and copies it hides
 static constexpr int64_t kMinHostChunk = 1 << 18;
+// result records of the host pipeline's last launch copied back with their count
+constexpr uint64_t kPatchFirst = 16384;
 
 
 static inline uint32_t next_pow2(uint64_t x) {
@@ -1360,7 +1364,7 @@ int ssw_context::host_pipeline(int32_t n, int64_t count, int64_t nchunks,
     // on; the chunks' rows and results have their place in one set of
     // whole-batch buffers
     const ssw_tuning &tn = tune[n];
-    const bool defer = tn.narrow_bits >= 8 && tn.narrow_bits < n && tn.spill_arena_lds > 0;
+    const bool defer = tn.narrow_bits >= 8 && tn.narrow_bits < n && tn.spill_arena_lds > 0 && nchunks <= 4;
     const uint64_t most = defer ? (uint64_t)count : (uint64_t)((count + nchunks - 1) / nchunks);
     const uint64_t out_row = 4u + (sync_out ? 1u : 0u) + (expanded_out ? 8u : 0u);
     const uint64_t half = align_up(most * out_row + 16, 256);
@@ -1378,7 +1382,9 @@ int ssw_context::host_pipeline(int32_t n, int64_t count, int64_t nchunks,
         if ((rc = ensure(deferred, 4ull * count, "deferred list"))) return rc;
         if ((rc = ensure(pipe_q, kQueueBytes, "work queue"))) return rc;
         if ((rc = ensure(pipe_list, 4ull * list_items, "spill list"))) return rc;
-        if ((rc = ensure_out(half + 16u * (uint64_t)(nchunks + 1)))) return rc;
+        if ((rc = ensure(pipe_patch, 16ull * (list_items + 1), "result records"))) return rc;
+        if ((rc = ensure_out(half + 16u * (uint64_t)(nchunks + 1) + 16u * (kPatchFirst + 1)))) return rc;
+        HIP_TRY(fill_u32((uint32_t *)pipe_patch.p, 4, 0u, stream), "clear");
         // set up before the first chunk's launches, on their stream
         HIP_TRY(init_queues((uint32_t *)pipe_q.p, list_items, stream, true), "init queues");
     } else {
@@ -1455,11 +1461,40 @@ int ssw_context::host_pipeline(int32_t n, int64_t count, int64_t nchunks,
         return SSW_OK;
     };
     if (defer) {
+        // chunk k's results are copied back (on the back stream) once its
+        // first two launches are done and handed to the caller when the host
+        // sees them there (pipe_ev[4 + k]); those of the automata the last
+        // launch takes are stale in that copy and are patched from records
+        // the gather kernel writes after the last launch
+        int64_t delivered = 0;
+        const auto deliver_done = [&](int64_t upto, bool wait) -> int {
+            while (delivered < upto) {
+                if (wait) {
+                    HIP_TRY(hipEventSynchronize(pipe_ev[4 + delivered]), "hipEventSynchronize");
+                } else {
+                    const hipError_t q = hipEventQuery(pipe_ev[4 + delivered]);
+                    if (q == hipErrorNotReady) return SSW_OK;
+                    if (q != hipSuccess) return hip_fail(q, "hipEventQuery");
+                }
+                const int64_t lo = chunk_lo(delivered), len = chunk_lo(delivered + 1) - lo;
+                parallel_ranges(len, 1 << 16, [&](int64_t a, int64_t b) {
+                    if (length_out) std::memcpy(length_out + lo + a, out_len(0) + lo + a, 4ull * (b - a));
+                    if (sync_out) std::memcpy(sync_out + lo + a, out_sync(0) + lo + a, (size_t)(b - a));
+                    if (expanded_out) std::memcpy(expanded_out + lo + a, out_exp(0) + lo + a, 8ull * (b - a));
+                });
+                call_done_base.fetch_add((uint64_t)len);  // the progress feed's in-flight count
+                ++delivered;
+            }
+            return SSW_OK;
+        };
         ClassSpec part;
         part.spill_queue[1] = (uint32_t *)pipe_q.p;
         part.spill_list[1] = (uint32_t *)pipe_list.p;
         for (int64_t k = 0; k < nchunks; ++k) {
-            while (filled[k].load(std::memory_order_acquire) < workers) std::this_thread::yield();
+            while (filled[k].load(std::memory_order_acquire) < workers) {
+                if ((rc = deliver_done(k, false))) return drain(rc);
+                std::this_thread::yield();
+            }
             if (bad[k].load() >= 0) {
                 const int64_t first = bad[k].load();
                 drain(0);
@@ -1483,6 +1518,17 @@ int ssw_context::host_pipeline(int32_t n, int64_t count, int64_t nchunks,
             part.list1_base = (uint32_t)lo;
             rc = solve_device(db, nullptr, false, nullptr, nullptr, heads + 16 * k, &part);
             if (rc) return drain(rc);
+            e = hipEventRecord(pipe_ev[2 + s], stream);
+            if (e == hipSuccess) e = hipStreamWaitEvent(back, pipe_ev[2 + s], 0);
+            if (e == hipSuccess && length_out)
+                e = hipMemcpyAsync(out_len(0) + lo, db.length, 4ull * len, hipMemcpyDeviceToHost, back);
+            if (e == hipSuccess && sync_out)
+                e = hipMemcpyAsync(out_sync(0) + lo, db.sync, (size_t)len, hipMemcpyDeviceToHost, back);
+            if (e == hipSuccess && expanded_out)
+                e = hipMemcpyAsync(out_exp(0) + lo, db.expanded, 8ull * len, hipMemcpyDeviceToHost, back);
+            if (e == hipSuccess) e = hipEventRecord(pipe_ev[4 + k], back);
+            if (e == hipSuccess) rc = deliver_done(k, false);
+            if (e != hipSuccess || rc) return drain(e != hipSuccess ? hip_fail(e, "pipelined chunk output") : rc);
         }
         join_all();
         DeviceBatch whole;
@@ -1495,22 +1541,39 @@ int ssw_context::host_pipeline(int32_t n, int64_t count, int64_t nchunks,
         whole.expanded = expanded_out ? (int64_t *)d_exp.p : nullptr;
         rc = solve_last_launch(whole, (uint32_t *)pipe_q.p, (uint32_t *)pipe_list.p, count, heads + 16 * nchunks);
         if (rc) return drain(rc);
-        hipError_t e = hipSuccess;
-        if (length_out) e = hipMemcpyAsync(out_len(0), d_len.p, 4ull * count, hipMemcpyDeviceToHost, stream);
-        if (e == hipSuccess && sync_out)
-            e = hipMemcpyAsync(out_sync(0), d_sync.p, (size_t)count, hipMemcpyDeviceToHost, stream);
-        if (e == hipSuccess && expanded_out)
-            e = hipMemcpyAsync(out_exp(0), d_exp.p, 8ull * count, hipMemcpyDeviceToHost, stream);
+        // the last launch's automata: their records, the first kPatchFirst of
+        // them in the same copy as the count
+        unsigned long long *rec = (unsigned long long *)(heads + align_up(16u * (uint64_t)(nchunks + 1), 16));
+        hipError_t e = gather_listed((const uint32_t *)pipe_q.p, (const uint32_t *)pipe_list.p, (uint32_t)list_items,
+                                     whole.length, whole.sync, whole.expanded, (unsigned long long *)pipe_patch.p, stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(rec, pipe_patch.p, 16u * (kPatchFirst + 1), hipMemcpyDeviceToHost, stream);
         if (e == hipSuccess) e = hipStreamSynchronize(stream);
         if (e != hipSuccess) return drain(hip_fail(e, "pipelined results"));
+        const uint64_t nrec = rec[0];
+        std::vector<unsigned long long> more;
+        if (nrec > kPatchFirst) {
+            more.resize(2 * (nrec - kPatchFirst));
+            e = hipMemcpy(more.data(), (uint8_t *)pipe_patch.p + 16u * (kPatchFirst + 1), 16u * (nrec - kPatchFirst),
+                          hipMemcpyDeviceToHost);
+            if (e != hipSuccess) return drain(hip_fail(e, "pipelined results"));
+        }
+        rc = deliver_done(nchunks, true);
+        e = hipStreamSynchronize(back);
+        join_all();
+        if (rc) return rc;
+        if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+        for (uint64_t q = 0; q < nrec; ++q) {
+            const unsigned long long w0 = q < kPatchFirst ? rec[2 + 2 * q] : more[2 * (q - kPatchFirst)];
+            const unsigned long long w1 = q < kPatchFirst ? rec[3 + 2 * q] : more[2 * (q - kPatchFirst) + 1];
+            const uint32_t a = (uint32_t)w0;
+            if ((int64_t)a >= count) return fail(SSW_ERR_INTERNAL, "result record of automaton %u", a);
+            if (length_out) length_out[a] = (int32_t)(uint32_t)(w0 >> 32);
+            if (sync_out) sync_out[a] = (uint8_t)(w1 >> 56);
+            if (expanded_out) expanded_out[a] = (int64_t)(w1 & 0x00FFFFFFFFFFFFFFull);
+        }
         float ms = 0;
         if (hipEventElapsedTime(&ms, ev0, ev1) == hipSuccess) last_ms = ms;  // the last launch
-        parallel_ranges(count, 1 << 16, [&](int64_t lo, int64_t hi) {
-            if (length_out) std::memcpy(length_out + lo, out_len(0) + lo, 4ull * (hi - lo));
-            if (sync_out) std::memcpy(sync_out + lo, out_sync(0) + lo, (size_t)(hi - lo));
-            if (expanded_out) std::memcpy(expanded_out + lo, out_exp(0) + lo, 8ull * (hi - lo));
-        });
-        call_done_base.fetch_add((uint64_t)count);  // the progress feed's in-flight count
         return read_heads(nchunks + 1, [&](int64_t k) { return k < nchunks ? chunk_lo(k) : (int64_t)0; });
     }
     for (int64_t k = 0; k < nchunks; ++k) {
