@@ -467,8 +467,9 @@ __global__ void reset_counters_kernel(uint32_t *deferred, unsigned long long *vi
 // `items` slots), as 16-byte records {index | length << 32, expanded | flag
 // << 56} appended after out[0..1] (out[0] counts them, zeroed by the caller):
 // the host pipeline copies these back after the last launch of a batch whose
-// other results it copied earlier.
-__global__ void gather_listed_kernel(const uint32_t *queue, const uint32_t *list, uint32_t items,
+// other results it copied earlier.  An index at or past `count` (none is) is
+// recorded without reading the results, for the host to report.
+__global__ void gather_listed_kernel(const uint32_t *queue, const uint32_t *list, uint32_t items, uint32_t count,
                                      const int32_t *length, const uint8_t *sync, const int64_t *expanded,
                                      unsigned long long *out) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -492,9 +493,10 @@ __global__ void gather_listed_kernel(const uint32_t *queue, const uint32_t *list
     base = __builtin_amdgcn_readlane(base, (int)__builtin_ctzll(m));
     if (have) {
         const uint64_t pos = 2u + 2ull * (base + below);
-        const uint64_t e = expanded ? (uint64_t)expanded[a] : 0ull;
-        const uint64_t f = sync ? (uint64_t)sync[a] : 0ull;
-        out[pos] = (uint64_t)a | ((uint64_t)(uint32_t)length[a] << 32);
+        const bool in = a < count;
+        const uint64_t e = in && expanded ? (uint64_t)expanded[a] : 0ull;
+        const uint64_t f = in && sync ? (uint64_t)sync[a] : 0ull;
+        out[pos] = (uint64_t)a | ((uint64_t)(uint32_t)(in ? length[a] : 0) << 32);
         out[pos + 1] = (e & 0x00FFFFFFFFFFFFFFull) | (f << 56);
     }
 }
@@ -2641,11 +2643,12 @@ hipError_t reset_counters(uint32_t *deferred, unsigned long long *violation, uns
     return hipGetLastError();
 }
 
-hipError_t gather_listed(const uint32_t *queue, const uint32_t *list, uint32_t items, const int32_t *length,
-                         const uint8_t *sync, const int64_t *expanded, unsigned long long *out, hipStream_t st) {
+hipError_t gather_listed(const uint32_t *queue, const uint32_t *list, uint32_t items, uint32_t count,
+                         const int32_t *length, const uint8_t *sync, const int64_t *expanded, unsigned long long *out,
+                         hipStream_t st) {
     if (items == 0) return hipSuccess;
-    hipLaunchKernelGGL(gather_listed_kernel, dim3((items + 255u) / 256u), dim3(256), 0, st, queue, list, items, length,
-                       sync, expanded, out);
+    hipLaunchKernelGGL(gather_listed_kernel, dim3((items + 255u) / 256u), dim3(256), 0, st, queue, list, items, count,
+                       length, sync, expanded, out);
     return hipGetLastError();
 }
 

@@ -40,8 +40,9 @@ hipError_t init_queues3(uint32_t *q0, uint32_t *q1, uint32_t *q2, int64_t count,
                         hipStream_t st);
 hipError_t reset_counters(uint32_t *deferred, unsigned long long *violation, unsigned long long *progress,
                           unsigned long long *launch_stats, uint32_t stat_words, hipStream_t st);
-hipError_t gather_listed(const uint32_t *queue, const uint32_t *list, uint32_t items, const int32_t *length,
-                         const uint8_t *sync, const int64_t *expanded, unsigned long long *out, hipStream_t st);
+hipError_t gather_listed(const uint32_t *queue, const uint32_t *list, uint32_t items, uint32_t count,
+                         const int32_t *length, const uint8_t *sync, const int64_t *expanded, unsigned long long *out,
+                         hipStream_t st);
 hipError_t noalloc_scratch(int device, void **out);
 std::mutex &noalloc_call_lock(int device);
 }  // namespace ssw
@@ -1545,7 +1546,8 @@ int ssw_context::host_pipeline(int32_t n, int64_t count, int64_t nchunks,
         // them in the same copy as the count
         unsigned long long *rec = (unsigned long long *)(heads + align_up(16u * (uint64_t)(nchunks + 1), 16));
         hipError_t e = gather_listed((const uint32_t *)pipe_q.p, (const uint32_t *)pipe_list.p, (uint32_t)list_items,
-                                     whole.length, whole.sync, whole.expanded, (unsigned long long *)pipe_patch.p, stream);
+                                     (uint32_t)count, whole.length, whole.sync, whole.expanded,
+                                     (unsigned long long *)pipe_patch.p, stream);
         if (e == hipSuccess)
             e = hipMemcpyAsync(rec, pipe_patch.p, 16u * (kPatchFirst + 1), hipMemcpyDeviceToHost, stream);
         if (e == hipSuccess) e = hipStreamSynchronize(stream);
