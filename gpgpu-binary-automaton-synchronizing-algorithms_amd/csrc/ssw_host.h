@@ -3,7 +3,9 @@
 
 #include "ssw.h"
 #include "ssw_internal.h"
+#include "ssw_pool.h"
 
+#include <algorithm>
 #include <atomic>
 #include <functional>
 #include <condition_variable>
@@ -294,6 +296,11 @@ struct ssw_context {
     int host_pipeline(int32_t n, int64_t count, int64_t nchunks, const std::function<int64_t(int64_t, int64_t)> &fill,
                       const std::function<int(int64_t)> &bad_at, int32_t *length_out, uint8_t *sync_out,
                       int64_t *expanded_out, int32_t bound, int64_t *violation_out);
+    // host worker threads of the host entry points
+    ssw::HostPool pool;
+    // fn over [0, count) in ranges of at least per_thread on the pool's
+    // threads (or on this one when the pool is busy or the count small)
+    void parallel_ranges(int64_t count, int64_t per_thread, const std::function<void(int64_t, int64_t)> &fn);
     // host staging for the int32 entry points: pinned, grown on demand
     void *h_stage = nullptr;
     size_t h_stage_bytes = 0;

@@ -983,18 +983,15 @@ hipError_t ssw_context::occupancy(int variant, int G, int block, size_t lds, int
 
 // Runs fn(lo, hi) over [0, count) on up to 16 host threads (one when the
 // range is short).
-template <class F>
-static void parallel_ranges(int64_t count, int64_t per_thread, F &&fn) {
-    const int64_t hw = std::max<int64_t>(1, (int64_t)std::thread::hardware_concurrency());
-    const int threads = (int)std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(hw, 16), count / per_thread));
-    if (threads == 1) {
+void ssw_context::parallel_ranges(int64_t count, int64_t per_thread,
+                                  const std::function<void(int64_t, int64_t)> &fn) {
+    const int threads = (int)std::max<int64_t>(1, std::min<int64_t>(pool.size(), count / per_thread));
+    if (threads == 1 || !pool.idle()) {
         fn((int64_t)0, count);
         return;
     }
-    std::vector<std::thread> ts;
-    for (int t = 0; t < threads; ++t)
-        ts.emplace_back([&fn, count, t, threads] { fn(count * t / threads, count * (t + 1) / threads); });
-    for (auto &th : ts) th.join();
+    pool.run([&fn, count, threads](int t) { fn(count * t / threads, count * (t + 1) / threads); }, threads);
+    pool.wait();
 }
 
 int ssw_context::ensure_stage(size_t bytes) {
@@ -1351,8 +1348,7 @@ int ssw_context::host_pipeline(int32_t n, int64_t count, int64_t nchunks,
     if (violation_out) *violation_out = -1;
     uint8_t *packed = (uint8_t *)h_stage;
     auto chunk_lo = [&](int64_t k) { return count * k / nchunks; };
-    const int64_t hw = std::max<int64_t>(1, (int64_t)std::thread::hardware_concurrency());
-    const int workers = (int)std::max<int64_t>(1, std::min<int64_t>(hw, 16) - 1);
+    const int workers = pool.size();
     std::vector<std::atomic<int>> filled(nchunks);
     std::vector<std::atomic<int64_t>> bad(nchunks);
     for (int64_t k = 0; k < nchunks; ++k) {
@@ -1422,12 +1418,8 @@ int ssw_context::host_pipeline(int32_t n, int64_t count, int64_t nchunks,
             filled[k].fetch_add(1, std::memory_order_release);
         }
     };
-    std::vector<std::thread> ts;
-    for (int t = 0; t < workers; ++t) ts.emplace_back(work, t);
-    const auto join_all = [&] {
-        for (auto &th : ts) th.join();
-        ts.clear();
-    };
+    pool.run(work, workers);
+    const auto join_all = [&] { pool.wait(); };
     const auto deliver = [&](int64_t k) -> int {  // chunk k's results to the caller
         const int s = (int)(k & 1);
         HIP_TRY(hipEventSynchronize(pipe_ev[4 + s]), "hipEventSynchronize");
@@ -1670,8 +1662,7 @@ int ssw_context::compute_i32(int32_t n, int64_t count, const int32_t *A, const i
         return rc ? rc : crc;
     }
     auto chunk_lo = [&](int64_t k) { return count * k / nchunks; };
-    const int64_t hw = std::max<int64_t>(1, (int64_t)std::thread::hardware_concurrency());
-    const int workers = (int)std::max<int64_t>(1, std::min<int64_t>(hw, 16) - (nchunks > 1 ? 1 : 0));
+    const int workers = pool.size();
     std::vector<std::atomic<int>> packed_parts(nchunks);
     std::vector<std::atomic<int64_t>> bad(nchunks);
     for (int64_t k = 0; k < nchunks; ++k) {
@@ -1700,15 +1691,8 @@ int ssw_context::compute_i32(int32_t n, int64_t count, const int32_t *A, const i
             packed_parts[k].fetch_add(1, std::memory_order_release);
         }
     };
-    std::vector<std::thread> ts;
-    if (workers > 1 || nchunks > 1)
-        for (int t = 0; t < workers; ++t) ts.emplace_back(work, t);
-    else
-        work(0);
-    auto join_all = [&] {
-        for (auto &th : ts) th.join();
-        ts.clear();
-    };
+    pool.run(work, workers);
+    auto join_all = [&] { pool.wait(); };
     static int32_t no_length;
     int64_t viol = -1;
     const auto bad_input = [&](int64_t first) {
