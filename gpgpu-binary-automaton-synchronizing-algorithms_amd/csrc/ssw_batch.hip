@@ -514,6 +514,7 @@ __global__ void gather_listed_kernel(const uint32_t *queue, const uint32_t *list
 template <class Grp, int T = 1, bool FAM = false>
 struct GroupSolver {
     static constexpr uint32_t G = Grp::kSize;
+    static constexpr int kWavesPerEu = 1;
     static constexpr uint32_t kFamBit = 1u << kFamBits;
     static constexpr uint32_t kTabBytes = FAM ? (uint32_t)kFamTableBytes : (uint32_t)kTableBytes;
     const BatchParams &P;
@@ -1063,6 +1064,9 @@ struct SmallSolver {
     static_assert(!WIT || Grp::kSize == 64, "witness words: one wavefront per automaton");
     static_assert(!FAM || (Grp::kSize == 64 && T == 1 && !BT && !WIT), "two-family encoding: one wavefront, T = 1");
     static constexpr uint32_t G = Grp::kSize;
+    // the witness launch of one entry per lane keeps the residency its LDS
+    // allows (six waves per SIMD) with its witness pass's registers
+    static constexpr int kWavesPerEu = (WIT && T == 1) ? 6 : 1;
     static constexpr uint32_t kGlobalRing = 65536;
     // one persistent loop per automaton when a group is a whole wavefront or
     // more (its control flow is uniform)
@@ -1218,7 +1222,7 @@ struct SmallSolver {
 
     // Takes the next automaton from the work queue and issues the loads of
     // its size and transitions; they are consumed by start().
-    __device__ void prefetch() {
+    __device__ __forceinline__ void prefetch() {
         const int64_t item = next_item();
         if (item < 0) {
             nx_ai = -1;
@@ -1838,6 +1842,78 @@ struct SmallSolver {
         else
             goodp[SSW_IDX(p, 65536u)] = on ? 1u : 0u;
     }
+    // witness() for a local history of at most kRegEntries * G subsets (most
+    // searches of the first launches, one entry per lane): lane r holds
+    // entries r + G u, u < kRegEntries, and their images, all looked up at once; a level's good
+    // flags are bits of `good`.  Per level the backward pass then costs one
+    // round of bitmap reads and one of marks, and per letter the forward pass
+    // one image lookup and a ballot over the registers (the LDS pass reads the
+    // history, its images and the flags again at every level and letter).
+    // A slot u takes part in level [s, e) only when s < (u + 1) G and
+    // e > u G, a test uniform over the wavefront.
+    static constexpr uint32_t kRegEntries = 2;
+    __device__ void witness_regs(int32_t L, const WitnessSpace &w) {
+        const uint32_t r = g.rank();
+        uint32_t x[kRegEntries], im[kRegEntries];
+#pragma unroll
+        for (uint32_t u = 0; u < kRegEntries; ++u) {
+            const uint32_t p = r + u * G;
+            x[u] = p < wr ? hist_at(w, p) : 0u;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kRegEntries; ++u) im[u] = image(x[u]);
+        uint32_t good = 0;
+        for (int32_t j = L - 1; j >= 1; --j) {
+            const uint32_t s = level_start(w, j), e = level_start(w, j + 1);
+            const bool last = j == L - 1;
+            uint32_t on = 0;
+#pragma unroll
+            for (uint32_t u = 0; u < kRegEntries; ++u) {
+                if (s < (u + 1u) * G && e > u * G) {
+                    const uint32_t p = r + u * G;
+                    const uint32_t a = im[u] & 0xFFFFu, b = im[u] >> 16;
+                    const bool ok = last ? (is_single(a) || is_single(b)) : (marked(a) || marked(b));
+                    if (p >= s && p < e && ok) on |= 1u << u;
+                }
+            }
+            lsync();
+#pragma unroll
+            for (uint32_t u = 0; u < kRegEntries; ++u)
+                if ((on >> u) & 1u) or_wg(l32(bm_a | ((x[u] >> 3) & 0x1FFCu)), 1u << (x[u] & 31u));
+            good |= on;
+            lsync();
+        }
+        glb_u8 *out = row();
+        uint32_t cur = Q, acc = 0;
+        for (int32_t k = 0; k < L; ++k) {
+            const uint32_t ic = image(cur);
+            const uint32_t a = ic & 0xFFFFu, b = ic >> 16;
+            bool take_a;
+            if (k == L - 1) {
+                take_a = is_single(a);
+            } else if (k == 0) {
+                take_a = marked(a);  // the bitmap holds level 1's good subsets
+            } else {
+                // a is taken when a(cur) is a good entry of level k + 1
+                const uint32_t s = level_start(w, k + 1), e = level_start(w, k + 2);
+                bool mine = false;
+#pragma unroll
+                for (uint32_t u = 0; u < kRegEntries; ++u) {
+                    const uint32_t p = r + u * G;
+                    mine |= p >= s && p < e && ((good >> u) & 1u) != 0u && x[u] == a;
+                }
+                take_a = g.any(mine);
+            }
+            if (!take_a) acc |= 1u << (k & 7);
+            cur = take_a ? a : b;
+            if ((k & 7) == 7 || k == L - 1) {
+                if (r == 0 && (k >> 3) < P.witness_stride) out[k >> 3] = (uint8_t)acc;
+                acc = 0;
+            }
+        }
+        zero_row((L + 7) >> 3);
+    }
+
     // The lexicographically smallest shortest reset word (a before b), the
     // word the reference's back-tracking yields (CPU.cs:262-269).  Backward,
     // level j = L-1 .. 1: a subset of level j is good when an image of it is a
@@ -1862,6 +1938,12 @@ struct SmallSolver {
         if (!w.local) {
             witness_hbm(L, w);
             return;
+        }
+        if constexpr (T == 1) {
+            if (wr <= kRegEntries * G) {
+                witness_regs(L, w);
+                return;
+            }
         }
         // a level of at most G entries (most of them) takes one entry per lane
         for (int32_t j = L - 1; j >= 1; --j) {
@@ -2439,7 +2521,7 @@ struct SmallSolver {
         if (done_local >= 256) flush();
     }
 
-    __device__ void flush() {
+    __device__ __forceinline__ void flush() {
         if (g.rank() == 0) {
             const uint32_t slot = blockIdx.x & (kCounterSlots - 1u);
             if (done_local) {
@@ -2519,7 +2601,8 @@ __device__ __forceinline__ void batch_body(const BatchParams &P) {
 }
 
 template <class Solver, class Grp>
-__global__ __launch_bounds__(256) void batch_kernel(BatchParams P) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Solver::kWavesPerEu))) void batch_kernel(
+    BatchParams P) {
     batch_body<Solver, Grp>(P);
 }
 
