@@ -1034,10 +1034,43 @@ int ssw_context::solve_host_packed(const ssw_batch *b, int mode, int64_t *violat
             [&](int64_t first) { return fail(SSW_ERR_ARG, "automaton %lld: a transition >= n=%d", (long long)first, n); },
             host_no_length ? nullptr : b->length, b->sync, b->expanded, b->bound, violation_out);
     }
+    // Calls with witness words are solved whole (their passes make every
+    // chunk's tail long); large ones still go through pinned memory both
+    // ways: the pool copies the caller's rows into the staging buffer while
+    // checking them, and the results come back into pinned memory and are
+    // copied out by the pool (the driver's copies from and to pageable
+    // memory run at a fraction of the pinned rate)
+    const bool bounce = mode == kModeBatch && b->uniform_n > 0 && !b->offset && b->witness && count >= kMinHostChunk;
+    const bool staged = bounce && !host_checked;  // (the int32 entry points pass rows already in the staging)
     // extent of the packed transitions and validation (first bad automaton)
     uint64_t extent = 0;
     if (host_checked && b->uniform_n > 0 && !b->offset) {
         extent = (uint64_t)count * 2u * (uint64_t)b->uniform_n;
+    } else if (staged) {
+        const int32_t n = b->uniform_n;
+        if (n < 1 || n > SSW_MAX_N) return fail(SSW_ERR_ARG, "n=%d out of [1,30]", n);
+        const size_t row = 2u * (size_t)n;
+        extent = (uint64_t)count * row;
+        int rc = ensure_stage((size_t)extent);
+        if (rc) return rc;
+        uint8_t *packed = (uint8_t *)h_stage;
+        std::mutex m;
+        int64_t bad = -1;
+        parallel_ranges(count, 1 << 16, [&](int64_t lo, int64_t hi) {
+            const uint8_t *src = b->delta + (size_t)lo * row;
+            std::memcpy(packed + (size_t)lo * row, src, (size_t)(hi - lo) * row);
+            uint8_t most = 0;
+            for (size_t q = 0, e = (size_t)(hi - lo) * row; q < e; ++q) most = std::max(most, src[q]);
+            if (most < (uint8_t)n) return;
+            for (int64_t i = lo; i < hi; ++i)
+                for (size_t q = 0; q < row; ++q)
+                    if (src[(size_t)(i - lo) * row + q] >= n) {
+                        std::lock_guard<std::mutex> lk(m);
+                        if (bad < 0 || i < bad) bad = i;
+                        return;
+                    }
+        });
+        if (bad >= 0) return fail(SSW_ERR_ARG, "automaton %lld: a transition >= n=%d", (long long)bad, n);
     } else {
         std::mutex m;
         int64_t bad = -1;
@@ -1070,7 +1103,9 @@ int ssw_context::solve_host_packed(const ssw_batch *b, int mode, int64_t *violat
     int rc;
     if ((rc = ensure(d_delta, extent, "delta"))) return rc;
     if ((rc = ensure(d_len, 4ull * count, "length"))) return rc;
-    HIP_TRY(hipMemcpyAsync(d_delta.p, b->delta, extent, hipMemcpyHostToDevice, stream), "H2D");
+    HIP_TRY(hipMemcpyAsync(d_delta.p, staged ? h_stage : (const void *)b->delta, extent, hipMemcpyHostToDevice,
+                           stream),
+            "H2D");
     DeviceBatch db;
     db.count = count;
     db.uniform_n = b->uniform_n;
@@ -1104,6 +1139,29 @@ int ssw_context::solve_host_packed(const ssw_batch *b, int mode, int64_t *violat
                             : solve_device(db, b->uniform_n > 0 ? nullptr : b->n_per, mode == kModePlain,
                                            violation_out);
     if (rc) return rc;
+    if (bounce) {
+        // results into pinned memory, then to the caller on the pool
+        const uint64_t wrow = (uint64_t)b->witness_stride;
+        const uint64_t o_sync = align_up(4ull * count, 64), o_exp = align_up(o_sync + (uint64_t)count, 64),
+                       o_wit = align_up(o_exp + 8ull * count, 64), total = o_wit + wrow * count;
+        if ((rc = ensure_out((size_t)total))) return rc;
+        uint8_t *h = (uint8_t *)h_out;
+        if (!host_no_length)
+            HIP_TRY(hipMemcpyAsync(h, db.length, 4ull * count, hipMemcpyDeviceToHost, stream), "D2H");
+        if (b->sync) HIP_TRY(hipMemcpyAsync(h + o_sync, db.sync, count, hipMemcpyDeviceToHost, stream), "D2H");
+        if (b->expanded)
+            HIP_TRY(hipMemcpyAsync(h + o_exp, db.expanded, 8ull * count, hipMemcpyDeviceToHost, stream), "D2H");
+        HIP_TRY(hipMemcpyAsync(h + o_wit, db.witness, wrow * count, hipMemcpyDeviceToHost, stream), "D2H");
+        HIP_TRY(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        parallel_ranges(count, 1 << 15, [&](int64_t lo, int64_t hi) {
+            const size_t k = (size_t)(hi - lo);
+            if (!host_no_length) std::memcpy(b->length + lo, h + 4ull * lo, 4 * k);
+            if (b->sync) std::memcpy(b->sync + lo, h + o_sync + lo, k);
+            if (b->expanded) std::memcpy(b->expanded + lo, h + o_exp + 8ull * lo, 8 * k);
+            std::memcpy(b->witness + wrow * lo, h + o_wit + wrow * lo, wrow * k);
+        });
+        return SSW_OK;
+    }
     if (!host_no_length)
         HIP_TRY(hipMemcpyAsync(b->length, db.length, 4ull * count, hipMemcpyDeviceToHost, stream), "D2H");
     if (b->sync) HIP_TRY(hipMemcpyAsync(b->sync, db.sync, count, hipMemcpyDeviceToHost, stream), "D2H");
@@ -1722,6 +1780,8 @@ int ssw_context::compute_i32(int32_t n, int64_t count, const int32_t *A, const i
         b.bound = bound;
         host_checked = true;
         host_no_length = length_out == nullptr;
+        // the last chunk is packed: the pool is free for the solve's copies
+        if (k == nchunks - 1) join_all();
         int64_t v = -1;
         rc = solve_entry(&b, SSW_HOST, kModeBatch, violation_out ? &v : nullptr);
         host_checked = false;

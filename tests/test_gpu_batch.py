@@ -74,6 +74,36 @@ def test_uniform_n16_witness(ctx):
     _check(n, A, B, r, witness_rows=range(len(A)), tag="n16w")
 
 
+def test_host_witness_calls_through_pinned_memory(ctx):
+    """Calls with witness words of 2^18 automata or more from host memory go
+    through pinned memory both ways (rows copied and checked by the host
+    pool into the staging buffer, results copied back into pinned memory and
+    out by the pool): lengths, flags and counts against the oracle for every
+    automaton, words against the oracle's back-tracking on a sample (with the
+    Cerny automata), both entry points, and a bad row reported by index."""
+    n = 16
+    count = (1 << 18) + 4_321
+    A, B = ssw.generate_uniform(1621, n, count)
+    ca, cb = ssw.generate_cerny(n)
+    cern = (0, 131_072, count - 1)
+    for i in cern:
+        A[i], B[i] = ca, cb
+    rows = ssw.pack_uniform(A, B).reshape(-1)
+    print("witness through pinned memory: solve_packed", flush=True)
+    r = ctx.solve_packed(rows, count, n, witness=True)
+    sample = list(cern) + list(np.random.default_rng(1622).choice(count, 400, replace=False))
+    _check(n, A, B, r, witness_rows=sample, tag="pinned witness rows")
+    print("witness through pinned memory: compute", flush=True)
+    r2 = ctx.compute(n, A, B, witness=True)
+    np.testing.assert_array_equal(r2.length, r.length)
+    np.testing.assert_array_equal(r2.expanded, r.expanded)
+    np.testing.assert_array_equal(r2.witness, r.witness)
+    bad = rows.copy().reshape(count, 2 * n)
+    bad[200_003, 5] = n
+    with pytest.raises(ssw.SswError, match="200003"):
+        ctx.solve_packed(bad.reshape(-1), count, n, witness=True)
+
+
 def test_mixed_sizes_small(ctx):
     rng = np.random.default_rng(5)
     ns = rng.integers(1, 17, size=3000).astype(np.uint8)
