@@ -1034,14 +1034,14 @@ int ssw_context::solve_host_packed(const ssw_batch *b, int mode, int64_t *violat
             [&](int64_t first) { return fail(SSW_ERR_ARG, "automaton %lld: a transition >= n=%d", (long long)first, n); },
             host_no_length ? nullptr : b->length, b->sync, b->expanded, b->bound, violation_out);
     }
-    // Calls with witness words are solved whole (their passes make every
-    // chunk's tail long); large ones still go through pinned memory both
-    // ways: the pool copies the caller's rows into the staging buffer while
-    // checking them, and the results come back into pinned memory and are
-    // copied out by the pool (the driver's copies from and to pageable
-    // memory run at a fraction of the pinned rate)
-    const bool bounce = mode == kModeBatch && b->uniform_n > 0 && !b->offset && b->witness && count >= kMinHostChunk;
-    const bool staged = bounce && !host_checked;  // (the int32 entry points pass rows already in the staging)
+    // Large batches solved here (witness words, n > 16, mixed sizes) go
+    // through pinned memory both ways: the pool copies uniform rows into the
+    // staging buffer while checking them, and the results come back into
+    // pinned memory and are copied out by the pool (the driver's copies from
+    // and to pageable memory run at a fraction of the pinned rate)
+    const bool bounce = mode == kModeBatch && count >= kMinHostChunk;
+    // (the int32 entry points pass rows already in the staging)
+    const bool staged = bounce && !host_checked && b->uniform_n > 0 && !b->offset;
     // extent of the packed transitions and validation (first bad automaton)
     uint64_t extent = 0;
     if (host_checked && b->uniform_n > 0 && !b->offset) {
@@ -1141,7 +1141,7 @@ int ssw_context::solve_host_packed(const ssw_batch *b, int mode, int64_t *violat
     if (rc) return rc;
     if (bounce) {
         // results into pinned memory, then to the caller on the pool
-        const uint64_t wrow = (uint64_t)b->witness_stride;
+        const uint64_t wrow = b->witness ? (uint64_t)b->witness_stride : 0u;
         const uint64_t o_sync = align_up(4ull * count, 64), o_exp = align_up(o_sync + (uint64_t)count, 64),
                        o_wit = align_up(o_exp + 8ull * count, 64), total = o_wit + wrow * count;
         if ((rc = ensure_out((size_t)total))) return rc;
@@ -1151,14 +1151,15 @@ int ssw_context::solve_host_packed(const ssw_batch *b, int mode, int64_t *violat
         if (b->sync) HIP_TRY(hipMemcpyAsync(h + o_sync, db.sync, count, hipMemcpyDeviceToHost, stream), "D2H");
         if (b->expanded)
             HIP_TRY(hipMemcpyAsync(h + o_exp, db.expanded, 8ull * count, hipMemcpyDeviceToHost, stream), "D2H");
-        HIP_TRY(hipMemcpyAsync(h + o_wit, db.witness, wrow * count, hipMemcpyDeviceToHost, stream), "D2H");
+        if (b->witness)
+            HIP_TRY(hipMemcpyAsync(h + o_wit, db.witness, wrow * count, hipMemcpyDeviceToHost, stream), "D2H");
         HIP_TRY(hipStreamSynchronize(stream), "hipStreamSynchronize");
         parallel_ranges(count, 1 << 15, [&](int64_t lo, int64_t hi) {
             const size_t k = (size_t)(hi - lo);
             if (!host_no_length) std::memcpy(b->length + lo, h + 4ull * lo, 4 * k);
             if (b->sync) std::memcpy(b->sync + lo, h + o_sync + lo, k);
             if (b->expanded) std::memcpy(b->expanded + lo, h + o_exp + 8ull * lo, 8 * k);
-            std::memcpy(b->witness + wrow * lo, h + o_wit + wrow * lo, wrow * k);
+            if (b->witness) std::memcpy(b->witness + wrow * lo, h + o_wit + wrow * lo, wrow * k);
         });
         return SSW_OK;
     }
