@@ -104,6 +104,35 @@ def test_host_witness_calls_through_pinned_memory(ctx):
         ctx.solve_packed(bad.reshape(-1), count, n, witness=True)
 
 
+def test_host_mixed_sizes_back_through_pinned_memory(ctx):
+    """A host batch of 2^18 automata or more of mixed sizes (rows at offsets,
+    n = 12..20, sizes interleaved) returns its results through pinned memory
+    and the host pool: every automaton against the oracle, per size."""
+    rng = np.random.default_rng(1623)
+    count = (1 << 18) + 777
+    ns = rng.integers(12, 21, size=count)
+    As, Bs = [], []
+    for n in range(12, 21):
+        k = int((ns == n).sum())
+        a, b = ssw.generate_uniform(1700 + n, n, k)
+        As.append(a)
+        Bs.append(b)
+    nxt = [0] * 21
+    A_list, B_list = [], []
+    for n in ns:
+        A_list.append(As[n - 12][nxt[n]])
+        B_list.append(Bs[n - 12][nxt[n]])
+        nxt[n] += 1
+    print("mixed sizes through pinned memory: solve", flush=True)
+    r = ctx.solve_mixed(ns, A_list, B_list)
+    for n in range(12, 21):
+        idx = np.nonzero(ns == n)[0]
+        L, S, E, _ = O.solve(n, As[n - 12], Bs[n - 12], threads=8)
+        np.testing.assert_array_equal(r.length[idx], L, err_msg=f"n={n}")
+        np.testing.assert_array_equal(r.sync[idx], S, err_msg=f"n={n}")
+        np.testing.assert_array_equal(r.expanded[idx], E, err_msg=f"n={n}")
+
+
 def test_mixed_sizes_small(ctx):
     rng = np.random.default_rng(5)
     ns = rng.integers(1, 17, size=3000).astype(np.uint8)
