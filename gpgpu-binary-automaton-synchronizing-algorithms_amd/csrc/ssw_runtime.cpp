@@ -1406,14 +1406,6 @@ int ssw_context::host_pipeline(int32_t n, int64_t count, int64_t nchunks,
                                int64_t *expanded_out, int32_t bound, int64_t *violation_out) {
     if (violation_out) *violation_out = -1;
     uint8_t *packed = (uint8_t *)h_stage;
-    auto chunk_lo = [&](int64_t k) { return count * k / nchunks; };
-    const int workers = pool.size();
-    std::vector<std::atomic<int>> filled(nchunks);
-    std::vector<std::atomic<int64_t>> bad(nchunks);
-    for (int64_t k = 0; k < nchunks; ++k) {
-        filled[k].store(0);
-        bad[k].store(-1);
-    }
     // n with the three-launch chain: every chunk runs the first two launches
     // as it arrives, and the last launch (the longest searches, whose tails
     // would otherwise end every chunk) runs once over what the chunks hand
@@ -1421,6 +1413,20 @@ int ssw_context::host_pipeline(int32_t n, int64_t count, int64_t nchunks,
     // whole-batch buffers
     const ssw_tuning &tn = tune[n];
     const bool defer = tn.narrow_bits >= 8 && tn.narrow_bits < n && tn.spill_arena_lds > 0 && nchunks <= 4;
+    // chunk boundaries; with deferral the first chunk has half the others'
+    // size, so the device starts sooner (the later chunks are packed while the
+    // earlier ones run): 4.02 -> 3.87 ms for the 1M n = 16 packed rows
+    auto chunk_lo = [count, nchunks, defer](int64_t k) -> int64_t {
+        if (!defer) return count * k / nchunks;
+        return k == 0 ? 0 : count * (2 * k - 1) / (2 * nchunks - 1);
+    };
+    const int workers = pool.size();
+    std::vector<std::atomic<int>> filled(nchunks);
+    std::vector<std::atomic<int64_t>> bad(nchunks);
+    for (int64_t k = 0; k < nchunks; ++k) {
+        filled[k].store(0);
+        bad[k].store(-1);
+    }
     const uint64_t most = defer ? (uint64_t)count : (uint64_t)((count + nchunks - 1) / nchunks);
     const uint64_t out_row = 4u + (sync_out ? 1u : 0u) + (expanded_out ? 8u : 0u);
     const uint64_t half = align_up(most * out_row + 16, 256);
