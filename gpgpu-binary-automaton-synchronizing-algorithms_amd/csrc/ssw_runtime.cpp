@@ -1413,11 +1413,11 @@ int ssw_context::host_pipeline(int32_t n, int64_t count, int64_t nchunks,
     // whole-batch buffers
     const ssw_tuning &tn = tune[n];
     const bool defer = tn.narrow_bits >= 8 && tn.narrow_bits < n && tn.spill_arena_lds > 0 && nchunks <= 4;
-    // chunk boundaries; with deferral the first chunk has half the others'
-    // size, so the device starts sooner (the later chunks are packed while the
-    // earlier ones run): 4.02 -> 3.87 ms for the 1M n = 16 packed rows
-    auto chunk_lo = [count, nchunks, defer](int64_t k) -> int64_t {
-        if (!defer) return count * k / nchunks;
+    // chunk boundaries: the first chunk has half the others' size, so the
+    // device starts sooner (the later chunks are packed while the earlier ones
+    // run): 4.02 -> 3.87 ms for the 1M n = 16 packed rows, about 3.05 -> 2.95 ms
+    // for the 2^21-automaton n = 13 Verify
+    auto chunk_lo = [count, nchunks](int64_t k) -> int64_t {
         return k == 0 ? 0 : count * (2 * k - 1) / (2 * nchunks - 1);
     };
     const int workers = pool.size();
@@ -1427,7 +1427,12 @@ int ssw_context::host_pipeline(int32_t n, int64_t count, int64_t nchunks,
         filled[k].store(0);
         bad[k].store(-1);
     }
-    const uint64_t most = defer ? (uint64_t)count : (uint64_t)((count + nchunks - 1) / nchunks);
+    // automata per device buffer set: the whole batch, or the largest chunk
+    uint64_t most = (uint64_t)count;
+    if (!defer) {
+        most = 0;
+        for (int64_t k = 0; k < nchunks; ++k) most = std::max<uint64_t>(most, (uint64_t)(chunk_lo(k + 1) - chunk_lo(k)));
+    }
     const uint64_t out_row = 4u + (sync_out ? 1u : 0u) + (expanded_out ? 8u : 0u);
     const uint64_t half = align_up(most * out_row + 16, 256);
     // the shared list of the last launch: its 64 queue segments each hold
